@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Benchmark of the COSEnu time-step hot path on one MI355X.
+
+Workload (the single-GPU production configuration): FV-WENO7 + RK4, FP64,
+nz = 65536 cells over z in [-1500, 1500], nvz = 128 velocity bins, CFL 0.4,
+periodic, mu = 1, omega = 0; synthetic flavour-unstable initial state
+(sigma_nu = 0.6, sigma_nubar = 0.53, alpha = 0.92, 1e-6 random-phase seed plus
+a Gaussian seed at z = 0).  A step is one full RK4 step of all 8 fields on
+all 8,388,608 cells; the metric is whole-job cell-updates per second with the
+state resident in HBM.
+
+Prints ONE JSON line with the throughput, the roofline of the dominant kernel
+(timed with HIP events on the library's stream) and a CPU baseline measured on
+this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 z·v cell-updates/s + HBM roofline %; surrogate vs solver P_ee sweep speedup"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_STEP = 1088           # y, stage input, accumulator: 3 x 8 FP64 per stage
+BYTES_PER_CELL_MID_STAGE = 320       # stages 2 and 3: read Y_s, y, acc; write acc, Y_s+1
+
+
+def cpu_baseline(seconds_target: float = 12.0):
+    """Time the reference CPU code (oracle/_ref, built from the reference's own
+    headers) or, where that build is absent, the oracle restatement, on a
+    z-subsampled slice of the workload: same dz, nvz = 128, nz = 2048."""
+    import numpy as np
+    import oracle as O
+    from cosenu_amd.initial import seeded_ic
+
+    threads = min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    dz = 3000.0 / 65536
+    nz = 2048
+    p = O.RunParams(nz=nz, nvz=128, z0=-nz * dz / 2, z1=nz * dz / 2, flags=O.VAC | O.COLL,
+                    omega=0.0, theta=1e-6)
+    G, bG, s = seeded_ic(p, 0.6, 0.53, 0.92, seed=1)
+    use_ref = O.ref_available("fv_vc")
+    if use_ref:
+        import tempfile
+        run = O.RefRun(p, tempfile.mkdtemp(prefix="cosenu_ref_"))
+        fn = lambda st, n: run.step(st, n)
+        kind = "reference"
+    else:
+        fn = lambda st, n: O.step(p, st, n, threads=threads)
+        kind = "port"
+    import contextlib
+    import io
+    sink = io.StringIO()
+    with contextlib.redirect_stdout(sink):
+        t0 = time.perf_counter()
+        fn(s, 1)
+        t1 = time.perf_counter()
+        per = max(t1 - t0, 1e-3)
+        n = max(1, min(40, int(seconds_target / per)))
+        t0 = time.perf_counter()
+        fn(s, n)
+        el = time.perf_counter() - t0
+    cells = nz * 128
+    return {"value": cells * n / el, "unit": "cell-updates/s", "cores": threads, "kind": kind,
+            "sample": f"{n} RK4 steps of FV-WENO7 (vac+coll) on nz={nz} x nvz=128, same dz as "
+                      f"the workload, {threads} OpenMP threads, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--scheme", choices=["fv", "fd"], default="fv")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c4"])
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the solver would shard as z-slabs with a periodic halo exchange; see DESIGN.md).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import cosenu_amd as C
+    from cosenu_amd.initial import workload
+
+    wl = workload(args.workload, mode=C.FAST, scheme=C.FV if args.scheme == "fv" else C.FD)
+    p = wl.params
+    G, bG, s0 = wl.ic()
+    cells = p.nz * p.nvz
+    sim = C.Simulation(p)
+    sim.set_profile(G, bG)
+    sim.upload(s0, C.STATE0)
+    sim.upload(s0)
+    del s0, G, bG
+
+    sim.step(args.warmup)
+    sim.sync()
+    sim.set_timing(True)   # HIP events around every stage launch on the library's stream
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    el = time.perf_counter() - t0
+    ms, cnt = sim.timing()
+    sim.set_timing(False)
+    value = cells * args.steps / el
+    ms_step = 1e3 * el / args.steps
+
+    # Dominant kernel: the middle-stage kernel (RK stages 2 and 3).
+    mid_ms = (ms[1] + ms[2]) / max(1, cnt[1] + cnt[2])
+    mid_bytes = BYTES_PER_CELL_MID_STAGE * cells
+    achieved = mid_bytes / (mid_ms * 1e-3) / 1e9
+    step_kernel_ms = sum(ms) / max(1, cnt[0])
+    step_gbs = BYTES_PER_CELL_STEP * cells / (step_kernel_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.scheme}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("mid_stage_hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{wl.name}: {args.scheme.upper()} RK4, nz={p.nz} on "
+                               f"[{p.z0:g},{p.z1:g}], nvz={p.nvz}, CFL={p.CFL}, periodic, "
+                               f"mu=1, omega=0, FAST mode", "cells": cells,
+                   "z_chunk": sim.info()["z_chunk"], "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": sim.stage_kernel_name(1),
+                     "kernel_avg_ms": mid_ms,
+                     "step_all_stages_GBs": step_gbs,
+                     "step_all_stages_frac": step_gbs / HBM_PEAK_GBS,
+                     "stage_avg_ms": [m / max(1, c) for m, c in zip(ms, cnt)]},
+    }
+    sim.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,310 @@
+// Layout conversion between the reference's host layouts and the device row
+// layout, and the on-device reductions behind the diagnostics
+// (src/analysis.hpp:27-185) and the P_ee snapshot.
+#include "common.hpp"
+
+namespace cosenu {
+namespace {
+
+constexpr int kT = 64; // transpose tile edge
+
+// plane (v rows of z) -> rows: dst[(z*nf + f)*VP + v] = src[v*stride + off + z];
+// padding bins (v >= nvz) are written as zero.
+__global__ __launch_bounds__(256) void k_plane_to_rows(double *dst, const double *src, int f, int nf,
+                                                       int nz, int nvz, int VP, ptrdiff_t stride,
+                                                       int off)
+{
+    __shared__ double t[kT][kT + 1];
+    const int z0 = blockIdx.x * kT, v0 = blockIdx.y * kT;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < kT; r += 4) {
+        int v = v0 + r, z = z0 + tx;
+        t[r][tx] = (v < nvz && z < nz) ? src[(ptrdiff_t)v * stride + off + z] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < kT; r += 4) {
+        int z = z0 + r, v = v0 + tx;
+        if (z < nz && v < VP)
+            dst[((size_t)z * nf + f) * VP + v] = t[tx][r];
+    }
+}
+
+// rows -> plane, optionally dividing by a second state (ratio != nullptr:
+// ee/ee0 for the P_ee snapshot).
+__global__ __launch_bounds__(256) void k_rows_to_plane(double *dst, const double *src,
+                                                       const double *ratio, int f, int nz, int nvz,
+                                                       int VP, ptrdiff_t stride, int off)
+{
+    __shared__ double t[kT][kT + 1];
+    const int z0 = blockIdx.x * kT, v0 = blockIdx.y * kT;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < kT; r += 4) {
+        int z = z0 + r, v = v0 + tx;
+        double x = 0.0;
+        if (z < nz && v < nvz) {
+            size_t q = ((size_t)z * 8 + f) * VP + v;
+            x = src[q];
+            if (ratio)
+                x = x / ratio[q];
+        }
+        t[r][tx] = x;
+    }
+    __syncthreads();
+    for (int r = ty; r < kT; r += 4) {
+        int v = v0 + r, z = z0 + tx;
+        if (v < nvz && z < nz)
+            dst[(ptrdiff_t)v * stride + off + z] = t[tx][r];
+    }
+}
+
+// interleaved (v, z, 8 fields) <-> rows.  A tile is 64 bins x 8 z cells: per
+// bin that is 64 contiguous doubles of the interleaved buffer.
+__global__ __launch_bounds__(256) void k_inter_to_rows(double *dst, const double *src, int nz,
+                                                       int nvz, int VP)
+{
+    __shared__ double t[kT][kT + 1];
+    const int z0 = blockIdx.x * 8, v0 = blockIdx.y * kT;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < kT; r += 4) {
+        int v = v0 + r, z = z0 + tx / 8;
+        t[r][tx] = (v < nvz && z < nz) ? src[((size_t)v * nz + z0) * 8 + tx] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < kT; r += 4) { // r = zz*8 + f
+        int z = z0 + r / 8, f = r % 8, v = v0 + tx;
+        if (z < nz && v < VP)
+            dst[((size_t)z * 8 + f) * VP + v] = t[tx][r];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rows_to_inter(double *dst, const double *src, int nz,
+                                                       int nvz, int VP)
+{
+    __shared__ double t[kT][kT + 1];
+    const int z0 = blockIdx.x * 8, v0 = blockIdx.y * kT;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < kT; r += 4) {
+        int z = z0 + r / 8, f = r % 8, v = v0 + tx;
+        t[r][tx] = (z < nz && v < nvz) ? src[((size_t)z * 8 + f) * VP + v] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < kT; r += 4) {
+        int v = v0 + r, z = z0 + tx / 8;
+        if (v < nvz && z < nz)
+            dst[((size_t)v * nz + z0) * 8 + tx] = t[tx][r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics.  Slots of the partial / final arrays:
+enum {
+    D_DPMAX = 0, D_DBPMAX, D_AVDP, D_AVBDP, D_NEE0, D_NBEE0, D_E1, D_E2, D_E3,
+    D_NUM, D_BNUM, D_DEN, D_BDEN, D_TR, D_BTR, D_LEP, D_BAD, D_COUNT
+};
+static_assert(D_COUNT <= kDiagSlots, "diag slots");
+
+__device__ inline bool is_max_slot(int s) { return s == D_DPMAX || s == D_DBPMAX; }
+
+__device__ inline double norm3(double a, double b, double c) { return fabs(sqrt(a * a + b * b + c * c)); }
+
+// One block per range of z rows; 256 threads stride over (z, v) cells.
+__global__ __launch_bounds__(256) void k_diag_partial(const double *y, const double *y0,
+                                                      const double *prof, int nz, int nvz, int VP,
+                                                      int rows_per_block, double *partials)
+{
+    double acc[D_COUNT];
+    for (int s = 0; s < D_COUNT; s++)
+        acc[s] = 0.0;
+    const int zb = blockIdx.x * rows_per_block;
+    const int ze = min(nz, zb + rows_per_block);
+    const long ncell = (long)(ze - zb) * nvz;
+    for (long c = threadIdx.x; c < ncell; c += blockDim.x) {
+        const int z = zb + (int)(c / nvz), v = (int)(c % nvz);
+        const size_t q = (size_t)z * 8 * VP + v;
+        const double G = prof[((size_t)z * 2 + 0) * VP + v], bG = prof[((size_t)z * 2 + 1) * VP + v];
+        const double ee = y[q], xx = y[q + VP], exr = y[q + 2 * VP], exi = y[q + 3 * VP];
+        const double bee = y[q + 4 * VP], bxx = y[q + 5 * VP], bexr = y[q + 6 * VP],
+                     bexi = y[q + 7 * VP];
+        const double ee0 = y0[q], xx0 = y0[q + VP], exr0 = y0[q + 2 * VP], exi0 = y0[q + 3 * VP];
+        const double bee0 = y0[q + 4 * VP], bxx0 = y0[q + 5 * VP], bexr0 = y0[q + 6 * VP],
+                     bexi0 = y0[q + 7 * VP];
+        // cal_pol (analysis.hpp:35-43): note P2 = -2 exi/G but bP2 = +2 bexi/bG.
+        const double P1 = 2.0 * exr / G, P2 = -2.0 * exi / G, P3 = (ee - xx) / G;
+        const double bP1 = 2.0 * bexr / bG, bP2 = 2.0 * bexi / bG, bP3 = (bee - bxx) / bG;
+        const double nP = norm3(P1, P2, P3), nbP = norm3(bP1, bP2, bP3);
+        const double nP0 = norm3(2.0 * exr0 / G, -2.0 * exi0 / G, (ee0 - xx0) / G);
+        const double nbP0 = norm3(2.0 * bexr0 / bG, 2.0 * bexi0 / bG, (bee0 - bxx0) / bG);
+        const double a = fabs(nP - nP0), b = fabs(nbP - nbP0);
+        acc[D_DPMAX] = fmax(acc[D_DPMAX], a);
+        acc[D_DBPMAX] = fmax(acc[D_DBPMAX], b);
+        acc[D_AVDP] += a * G;
+        acc[D_AVBDP] += b * bG;
+        acc[D_NEE0] += G;
+        acc[D_NBEE0] += bG;
+        acc[D_E1] += G * P1 - bG * bP1; // cal_Mn (analysis.hpp:60-62), L_0 = 1
+        acc[D_E2] += G * P2 - bG * bP2;
+        acc[D_E3] += G * P3 - bG * bP3;
+        acc[D_NUM] += ee;
+        acc[D_BNUM] += bee;
+        acc[D_DEN] += ee0;
+        acc[D_BDEN] += bee0;
+        acc[D_TR] += ee + xx;
+        acc[D_BTR] += bee + bxx;
+        acc[D_LEP] += (ee - xx) - (bee - bxx);
+        const bool fin = isfinite(ee) && isfinite(xx) && isfinite(exr) && isfinite(exi) &&
+                         isfinite(bee) && isfinite(bxx) && isfinite(bexr) && isfinite(bexi);
+        acc[D_BAD] += fin ? 0.0 : 1.0;
+        if (!(a == a)) // NaN in |dP| must surface in the maximum too
+            acc[D_DPMAX] = a;
+    }
+    __shared__ double red[256];
+    for (int s = 0; s < D_COUNT; s++) {
+        red[threadIdx.x] = acc[s];
+        __syncthreads();
+        for (int w = 128; w >= 1; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                double o = red[threadIdx.x + w];
+                red[threadIdx.x] = is_max_slot(s) ? ((o > red[threadIdx.x] || o != o) ? o : red[threadIdx.x])
+                                                  : red[threadIdx.x] + o;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            partials[(size_t)blockIdx.x * kDiagSlots + s] = red[0];
+        __syncthreads();
+    }
+}
+
+// Fixed-order combination of the partials (deterministic run to run).
+__global__ void k_diag_final(const double *partials, int npart, double *out)
+{
+    const int s = threadIdx.x;
+    if (s >= D_COUNT)
+        return;
+    double x = 0.0;
+    for (int p = 0; p < npart; p++) {
+        double o = partials[(size_t)p * kDiagSlots + s];
+        x = is_max_slot(s) ? ((o > x || o != o) ? o : x) : x + o;
+    }
+    out[s] = x;
+}
+
+// Per-bin z sums of ee, ee0, bee, bee0 over a z range per block.
+__global__ __launch_bounds__(256) void k_surv_partial(const double *y, const double *y0, int nz,
+                                                      int nvz, int VP, int rows_per_block,
+                                                      double *partials)
+{
+    const int v = blockIdx.y * blockDim.x + threadIdx.x;
+    if (v >= VP)
+        return;
+    const int zb = blockIdx.x * rows_per_block, ze = min(nz, zb + rows_per_block);
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    for (int z = zb; z < ze; z++) {
+        const size_t q = (size_t)z * 8 * VP + v;
+        s0 += y[q];
+        s1 += y0[q];
+        s2 += y[q + 4 * VP];
+        s3 += y0[q + 4 * VP];
+    }
+    double *o = partials + (size_t)blockIdx.x * 4 * VP;
+    o[v] = s0;
+    o[VP + v] = s1;
+    o[2 * VP + v] = s2;
+    o[3 * VP + v] = s3;
+}
+
+__global__ void k_surv_final(const double *partials, int npart, int VP, double *out)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= VP)
+        return;
+    double s[4] = {0, 0, 0, 0};
+    for (int p = 0; p < npart; p++)
+        for (int k = 0; k < 4; k++)
+            s[k] += partials[((size_t)p * 4 + k) * VP + v];
+    out[v] = s[0] / s[1];
+    out[VP + v] = s[2] / s[3];
+}
+
+int rows_per_block(int nz, int npart) { return (nz + npart - 1) / npart; }
+
+} // namespace
+
+hipError_t launch_import_plane(double *dst, const double *src, int f, int nz, int nvz, int VP,
+                               ptrdiff_t stride, int off, hipStream_t s)
+{
+    dim3 g((nz + kT - 1) / kT, (VP + kT - 1) / kT);
+    hipLaunchKernelGGL(k_plane_to_rows, g, dim3(256), 0, s, dst, src, f, 8, nz, nvz, VP, stride, off);
+    return hipGetLastError();
+}
+
+hipError_t launch_import_profile(double *dst, const double *G, const double *bG, int nz, int nvz,
+                                 int VP, ptrdiff_t stride, int off, hipStream_t s)
+{
+    dim3 g((nz + kT - 1) / kT, (VP + kT - 1) / kT);
+    hipLaunchKernelGGL(k_plane_to_rows, g, dim3(256), 0, s, dst, G, 0, 2, nz, nvz, VP, stride, off);
+    hipLaunchKernelGGL(k_plane_to_rows, g, dim3(256), 0, s, dst, bG, 1, 2, nz, nvz, VP, stride, off);
+    return hipGetLastError();
+}
+
+hipError_t launch_export_plane(double *dst, const double *src, int f, int nz, int nvz, int VP,
+                               ptrdiff_t stride, int off, hipStream_t s)
+{
+    dim3 g((nz + kT - 1) / kT, (VP + kT - 1) / kT);
+    hipLaunchKernelGGL(k_rows_to_plane, g, dim3(256), 0, s, dst, src, (const double *)nullptr, f, nz,
+                       nvz, VP, stride, off);
+    return hipGetLastError();
+}
+
+hipError_t launch_pee_plane(double *dst, const double *y, const double *y0, int nz, int nvz, int VP,
+                            hipStream_t s)
+{
+    dim3 g((nz + kT - 1) / kT, (VP + kT - 1) / kT);
+    hipLaunchKernelGGL(k_rows_to_plane, g, dim3(256), 0, s, dst, y, y0, 0, nz, nvz, VP,
+                       (ptrdiff_t)nz, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_import_interleaved(double *dst, const double *src, int nz, int nvz, int VP,
+                                     hipStream_t s)
+{
+    dim3 g((nz + 7) / 8, (VP + kT - 1) / kT);
+    hipLaunchKernelGGL(k_inter_to_rows, g, dim3(256), 0, s, dst, src, nz, nvz, VP);
+    return hipGetLastError();
+}
+
+hipError_t launch_export_interleaved(double *dst, const double *src, int nz, int nvz, int VP,
+                                     hipStream_t s)
+{
+    dim3 g((nz + 7) / 8, (VP + kT - 1) / kT);
+    hipLaunchKernelGGL(k_rows_to_inter, g, dim3(256), 0, s, dst, src, nz, nvz, VP);
+    return hipGetLastError();
+}
+
+int diag_partials_needed(int nz) { return nz < 1024 ? nz : 1024; }
+
+hipError_t launch_diagnostics(const double *y, const double *y0, const double *prof, int nz,
+                              int nvz, int VP, double *partials, int npart, double *out,
+                              hipStream_t s)
+{
+    int rpb = rows_per_block(nz, npart);
+    int nb = (nz + rpb - 1) / rpb;
+    hipLaunchKernelGGL(k_diag_partial, dim3(nb), dim3(256), 0, s, y, y0, prof, nz, nvz, VP, rpb,
+                       partials);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, partials, nb, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_survival_profile(const double *y, const double *y0, int nz, int nvz, int VP,
+                                   double *partials, int npart, double *out, hipStream_t s)
+{
+    // partials must hold npart*4*VP doubles
+    int rpb = rows_per_block(nz, npart);
+    int nb = (nz + rpb - 1) / rpb;
+    dim3 g(nb, (VP + 255) / 256);
+    hipLaunchKernelGGL(k_surv_partial, g, dim3(256), 0, s, y, y0, nz, nvz, VP, rpb, partials);
+    hipLaunchKernelGGL(k_surv_final, dim3((VP + 255) / 256), dim3(256), 0, s, partials, nb, VP, out);
+    return hipGetLastError();
+}
+
+} // namespace cosenu

@@ -1,0 +1,86 @@
+// Shared definitions of the COSEnu MI355X kernels.
+//
+// Device state layout ("row layout"): for every z cell one row of 8 fields,
+// each field a run of VP doubles over the velocity bins, VP = nvz rounded up
+// to a multiple of 64 (one wavefront of lanes).  Element (z, f, v) lives at
+//     ((size_t)z * 8 + f) * VP + v.
+// A 64-lane load of one field at one z is one contiguous 512-byte segment.
+// Bins v >= nvz are padding, kept at zero and never written by the stage
+// kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace cosenu {
+
+enum Field { EE = 0, XX, EXR, EXI, BEE, BXX, BEXR, BEXI };
+
+constexpr int kFlagVac = 1, kFlagMat = 2, kFlagColl = 4, kFlagAdvecOff = 8, kFlagKO5 = 16;
+
+__host__ __device__ inline size_t row_index(int z, int f, int VP) { return ((size_t)z * 8 + f) * VP; }
+
+// Everything a stage kernel needs.  Stage s in 1..4 evaluates K = RHS(yin)
+// and applies the reference's RK4 buffer algebra (nuosc.hpp:320-348):
+//   s=1: acc = K;            yout = y + a*K        (a = 0.5 dt)
+//   s=2: acc = acc + 2.0*K;  yout = y + a*K        (a = 0.5 dt)
+//   s=3: acc = acc + 2.0*K;  yout = y + a*K        (a = dt)
+//   s=4:                     yout = y + a*(K+acc)  (a = dt/6, yout == y)
+struct StageArgs {
+    const double *yin; // stage input Y_s (read with its z-neighbours)
+    const double *y;   // state at the start of the step (pointwise)
+    double *acc;       // RK accumulator
+    double *yout;      // stage output
+    const double *vz;  // [VP] velocity of each bin (padding bins: 0)
+    const double *Hm;  // [nz] matter potential (MAT only)
+    int nz, nvz, VP, flags, bc, scheme;
+    double a;          // stage coefficient (see above)
+    double dz, dv, ct, st, pmo, mu, ko;
+};
+
+struct RhsArgs { // plain RHS evaluation: out = RHS(in), for calRHS
+    const double *in;
+    double *out;
+    const double *vz, *Hm;
+    int nz, nvz, VP, flags, bc, scheme;
+    double dz, dv, ct, st, pmo, mu, ko;
+};
+
+// Launchers (defined in the .hip files).
+hipError_t launch_stage_strict(const StageArgs &a, int stage, hipStream_t s);
+hipError_t launch_rhs_strict(const RhsArgs &a, hipStream_t s);
+const char *stage_strict_kernel_name(int stage);
+
+// FAST mode: returns hipErrorInvalidValue for unsupported shapes.
+hipError_t launch_stage_fast(const StageArgs &a, int stage, int z_chunk, hipStream_t s);
+hipError_t launch_rhs_fast(const RhsArgs &a, int z_chunk, hipStream_t s);
+const char *stage_fast_kernel_name(int VP, int stage);
+int fast_supported(int VP, int bc);
+
+// Layout conversion and reductions (aux.hip).
+hipError_t launch_import_plane(double *dst, const double *src, int f, int nz, int nvz, int VP,
+                               ptrdiff_t row_stride, int row_offset, hipStream_t s);
+hipError_t launch_export_plane(double *dst, const double *src, int f, int nz, int nvz, int VP,
+                               ptrdiff_t row_stride, int row_offset, hipStream_t s);
+hipError_t launch_import_interleaved(double *dst, const double *src, int nz, int nvz, int VP,
+                                     hipStream_t s);
+hipError_t launch_export_interleaved(double *dst, const double *src, int nz, int nvz, int VP,
+                                     hipStream_t s);
+// G/bG planes -> profile rows [z][2][VP]
+hipError_t launch_import_profile(double *dst, const double *G, const double *bG, int nz, int nvz,
+                                 int VP, ptrdiff_t row_stride, int row_offset, hipStream_t s);
+
+constexpr int kDiagSlots = 20;
+// Partial sums of the diagnostics over z-blocks, then a fixed-order final
+// reduction: out[kDiagSlots] on the device.
+hipError_t launch_diagnostics(const double *y, const double *y0, const double *prof, int nz,
+                              int nvz, int VP, double *partials, int npart, double *out,
+                              hipStream_t s);
+int diag_partials_needed(int nz);
+hipError_t launch_survival_profile(const double *y, const double *y0, int nz, int nvz, int VP,
+                                   double *partials, int npart, double *out, hipStream_t s);
+hipError_t launch_pee_plane(double *dst, const double *y, const double *y0, int nz, int nvz,
+                            int VP, hipStream_t s);
+
+} // namespace cosenu

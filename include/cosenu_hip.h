@@ -1,0 +1,172 @@
+/*
+ * cosenu_hip.h -- C ABI of the MI355X (gfx950) implementation of COSEnu's
+ * time-step hot path.
+ *
+ * One context (cosenu_ctx) = one simulation state on one GPU, the role of the
+ * reference's `class NuOsc` (src/nuosc.hpp:7-191).  Every entry point returns
+ * 0 on success or a negative cosenu_status; the message of the last failure
+ * on the calling thread is cosenu_last_error().  Nothing here calls exit().
+ *
+ * Arithmetic is IEEE FP64 throughout.  Two modes:
+ *   COSENU_STRICT  the reference's operation order, true divisions, the
+ *                  O(nvz) collective loop, no fused multiply-add: bit-identical
+ *                  to the reference CPU build (rhs_fv.hpp / rhs_fd.hpp).
+ *   COSENU_FAST    fused streaming stage kernels: one-sided WENO7 with a single
+ *                  division, the collective integral from two velocity moments
+ *                  per z, fused multiply-add.  Agrees with STRICT normwise.
+ *
+ * Host arrays in the reference's layout are accepted directly:
+ *   planes  8 arrays (ee, xx, ex_re, ex_im, bee, bxx, bex_re, bex_im), each
+ *           nvz rows; row i starts at i*row_stride and holds nz values from
+ *           offset row_offset.  The reference's FieldVar is row_stride =
+ *           nz+2*gz, row_offset = gz (structures.hpp:5-37, nuosc.hpp:145-148).
+ *   interleaved  for i in nvz, for j in nz: 8 doubles in the field order above,
+ *           the body of state.bin / state0.bin (snaps.hpp:20-35, 54-68).
+ * The device layout is internal and never exposed.
+ *
+ * Threading: a context is used by one host thread at a time; distinct
+ * contexts are independent.  cosenu_step() is asynchronous (it only enqueues
+ * work on the context's stream); the calls that return data synchronise.
+ */
+#ifndef COSENU_HIP_H
+#define COSENU_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COSENU_ABI_VERSION 1
+
+typedef enum {
+    COSENU_OK = 0,
+    COSENU_EINVAL = -1,   /* bad argument / unsupported combination */
+    COSENU_ENOMEM = -2,   /* device or host allocation failed */
+    COSENU_EHIP = -3,     /* HIP runtime error */
+    COSENU_ESTATE = -4,   /* call out of order (e.g. no profile set) */
+    COSENU_ENOTSUP = -5   /* feature not available in this mode */
+} cosenu_status;
+
+/* Scheme (reference: FV / FD preset, main.cpp:46-50). */
+enum { COSENU_FV = 0, COSENU_FD = 1 };
+/* Boundary (reference: PERIODIC_BC / OPEN_BC preset, nuosc.hpp:212-278). */
+enum { COSENU_PERIODIC = 0, COSENU_OPEN = 1 };
+/* Physics switches (reference presets VAC_OSC_ON, MAT_OSC_ON, COLL_OSC_ON,
+ * ADVEC_OFF; COSENU_KO5 = FD without KO_ORD_3, i.e. 5th-order dissipation). */
+enum { COSENU_VAC = 1, COSENU_MAT = 2, COSENU_COLL = 4, COSENU_ADVEC_OFF = 8, COSENU_KO5 = 16 };
+enum { COSENU_STRICT = 0, COSENU_FAST = 1 };
+/* Which state a transfer addresses. */
+enum { COSENU_STATE = 0,   /* v_stat: the evolving state */
+       COSENU_STATE0 = 1   /* v_stat0: the t=0 state used by the diagnostics */ };
+
+/* Run parameters: the keys of the reference's job.config (parser.hpp:46-97)
+ * plus the compile-time presets as runtime fields.  gz is accepted for
+ * compatibility; the device path keeps no ghost cells.  ko is the
+ * Kreiss-Oliger coefficient (nuosc.hpp:60, 0.1 in the reference). */
+typedef struct cosenu_params {
+    int scheme, bc, flags, nz, nvz, gz;
+    double z0, z1, v0, v1, CFL;
+    double pmo, omega, theta, mu, ko;
+    int mode;   /* COSENU_STRICT or COSENU_FAST */
+    int device; /* HIP device ordinal */
+} cosenu_params;
+
+/* Derived grid constants, as NuOsc's constructor computes them
+ * (nuosc.hpp:78-96, 132-143). */
+typedef struct cosenu_info {
+    double dz, dv, dt, ct, st;
+    double phy_time;
+    long long steps_done;
+    size_t device_bytes; /* device memory held by the context */
+    int z_chunk;         /* FAST mode: z cells per streaming work item */
+} cosenu_info;
+
+/* Diagnostics (analysis.hpp:3-185) plus conservation checks.  All sums are
+ * over the interior cells times dz*dv like the reference's. */
+typedef struct cosenu_diag {
+    double dP_max;    /* max over cells of | |P|-|P0| | for nu and nubar (dcon) */
+    double dP_avg;    /* G-weighted mean of | |P|-|P0| | (dcon) */
+    double dbP_avg;   /* bG-weighted mean for nubar (dcon) */
+    double M0_norm;   /* |M0| (cal_Mn, n = 0) */
+    double M0[3];     /* M0 components e1, e2, e3 */
+    double Pee;       /* <P_ee>  = sum ee / sum ee0 (surv_prob) */
+    double Pbee;      /* <P_bee> = sum bee / sum bee0 */
+    double trace;     /* sum (ee+xx) dz dv */
+    double btrace;    /* sum (bee+bxx) dz dv */
+    double lepton;    /* sum [(ee-xx)-(bee-bxx)] dz dv (= M0 e3) */
+    double nonfinite; /* number of non-finite state values (0 when healthy) */
+} cosenu_diag;
+
+typedef struct cosenu_ctx cosenu_ctx;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *cosenu_last_error(void);
+int cosenu_abi_version(void);
+
+/* NuOsc::NuOsc + set_vac_pars + set_collective_pars (nuosc.hpp:47-143).
+ * Allocates the device buffers; the state starts at zero. */
+int cosenu_create(const cosenu_params *p, cosenu_ctx **out);
+/* NuOsc::~NuOsc (nuosc.hpp:119-130). */
+int cosenu_destroy(cosenu_ctx *ctx);
+int cosenu_get_info(cosenu_ctx *ctx, cosenu_info *info);
+
+/* The angular profiles G0->G, G0->bG (structures.hpp:124-137), set by
+ * NuOsc::initialize (initialize.hpp:218-219) or NuOsc::read_G0
+ * (snaps.hpp:161-183).  Planes of nvz rows as described above. */
+int cosenu_set_profile(cosenu_ctx *ctx, const double *G, const double *bG, ptrdiff_t row_stride,
+                       int row_offset);
+
+/* State transfers.  `which` = COSENU_STATE or COSENU_STATE0.  Uploading
+ * COSENU_STATE replaces the v_stat that initialize/read_state fill
+ * (initialize.hpp:221-229, snaps.hpp:75-124); COSENU_STATE0 replaces the
+ * v_stat0 copy (main.cpp:131-150). */
+int cosenu_upload_planes(cosenu_ctx *ctx, int which, const double *const planes[8],
+                         ptrdiff_t row_stride, int row_offset);
+int cosenu_download_planes(cosenu_ctx *ctx, int which, double *const planes[8],
+                           ptrdiff_t row_stride, int row_offset);
+int cosenu_upload_interleaved(cosenu_ctx *ctx, int which, const double *buf);
+int cosenu_download_interleaved(cosenu_ctx *ctx, int which, double *buf);
+
+/* NuOsc::step_rk4 (nuosc.hpp:320-348) nsteps times.  Enqueues only. */
+int cosenu_step(cosenu_ctx *ctx, int nsteps);
+/* Block until all work of the context has finished. */
+int cosenu_sync(cosenu_ctx *ctx);
+
+/* out = calRHS(v_stat) (rhs_fv.hpp:115-236 / rhs_fd.hpp:6-148), written into
+ * host planes.  Synchronous; for tests and for callers of calRHS. */
+int cosenu_rhs(cosenu_ctx *ctx, double *const out[8], ptrdiff_t row_stride, int row_offset);
+
+/* analyse + surv_prob (analysis.hpp:3-185) on the current state against
+ * v_stat0 and the profiles, reduced on the device.  Synchronous. */
+int cosenu_diagnostics(cosenu_ctx *ctx, cosenu_diag *d);
+/* Angle-resolved survival probability: pee[i] = sum_z ee(z,v_i) /
+ * sum_z ee0(z,v_i), pbee likewise for nubar.  Arrays of nvz.  Synchronous. */
+int cosenu_survival_profile(cosenu_ctx *ctx, double *pee, double *pbee);
+
+/* Snapshots without stopping the stepping: computes P_ee(z,v) = ee/ee0 into
+ * a device buffer on the compute stream, then copies it on a second stream
+ * into `host` (nvz rows of nz doubles, v-major like dump_rho's data).  `host`
+ * must come from cosenu_host_alloc.  cosenu_snapshot_wait blocks until the
+ * copy has landed. */
+int cosenu_snapshot_pee_async(cosenu_ctx *ctx, double *host);
+int cosenu_snapshot_wait(cosenu_ctx *ctx);
+void *cosenu_host_alloc(size_t bytes);
+void cosenu_host_free(void *p);
+
+/* Per-kernel timing with HIP events recorded around every stage launch on
+ * the context's stream.  ms[s] = total milliseconds of RK stage s (0..3)
+ * since timing was enabled, counts[s] = launches timed.  Enabling clears. */
+int cosenu_set_timing(cosenu_ctx *ctx, int enable);
+int cosenu_get_timing(cosenu_ctx *ctx, double ms[4], long long counts[4]);
+
+/* Name of the HIP kernel that runs RK stage s (0..3) in this context, as
+ * profilers report it. */
+const char *cosenu_stage_kernel_name(cosenu_ctx *ctx, int stage);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COSENU_HIP_H */

@@ -62,7 +62,7 @@ def cpu_baseline(seconds_target: float = 12.0):
         fn(s, 1)
         t1 = time.perf_counter()
         per = max(t1 - t0, 1e-3)
-        n = max(1, min(40, int(seconds_target / per)))
+        n = max(1, min(400, int(seconds_target / per)))
         t0 = time.perf_counter()
         fn(s, n)
         el = time.perf_counter() - t0

@@ -28,6 +28,7 @@
 // iterations later, so the cross-wave exchange never waits on fresh data.
 #include "common.hpp"
 #include <cstdio>
+#include <cstdlib>
 
 namespace cosenu {
 namespace {
@@ -295,6 +296,287 @@ __global__ __launch_bounds__(256) void cosenu_fast_stage(StageArgs A, int L, int
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split-lane variant.  The 8 fields of a velocity bin are shared by two lanes
+// of one wavefront: lanes 0-31 carry (ee, xx, ex_re, ex_im) of 32 bins and
+// lanes 32-63 carry (bee, bxx, bex_re, bex_im) of the same bins.  The nu and
+// nubar halves of the right-hand side couple only through the per-z moments,
+// which travel through the LDS ring anyway, so every lane needs a window of
+// 4 fields (64 VGPRs) instead of 8.  A team is VP/32 wavefronts; with nvz a
+// multiple of 64 every wavefront's bins share one sign of v, which selects the
+// upwind stencil order with a wave-uniform branch instead of per-lane selects.
+//
+// With sigma = +1 for the nu lanes and -1 for the nubar lanes, and
+// (p, q, r, s) = (ee, xx, ex_re, ex_im) or (bee, bxx, bex_re, bex_im), the
+// local terms of rhs_fv.hpp:164-232 read
+//   K_p = -2 st s + 2 (r S - sigma s R)           K_q = -K_p (local part)
+//   K_r = -2 ct s + sigma h s + (q - p) S + sigma s D
+//   K_s =  2 ct r + st (p - q) - sigma h r + sigma (p - q) R - sigma r D
+// with S, R, D the collective integrals mu dv (F0 - v F1) and h = Hm(z).
+
+// Sum of 6 per-lane values over the 64 lanes: a butterfly that halves the
+// number of values carried at each of the first three levels.  Returns the
+// total of value ((l>>5)&1)*4 + ((l>>4)&1)*2 + ((l>>3)&1) in lane l (values
+// 6 and 7 are zero padding).
+__device__ inline double wave_sum6(const double m6[6], int lane)
+{
+    double m[8] = {m6[0], m6[1], m6[2], m6[3], m6[4], m6[5], 0.0, 0.0};
+    const bool b5 = (lane >> 5) & 1, b4 = (lane >> 4) & 1, b3 = (lane >> 3) & 1;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double send = b5 ? m[k] : m[k + 4];
+        double keep = b5 ? m[k + 4] : m[k];
+        m[k] = keep + __shfl_xor(send, 32, 64);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        double send = b4 ? m[k] : m[k + 2];
+        double keep = b4 ? m[k + 2] : m[k];
+        m[k] = keep + __shfl_xor(send, 16, 64);
+    }
+    {
+        double send = b3 ? m[0] : m[1];
+        double keep = b3 ? m[1] : m[0];
+        m[0] = keep + __shfl_xor(send, 8, 64);
+    }
+    m[0] += __shfl_xor(m[0], 4, 64);
+    m[0] += __shfl_xor(m[0], 2, 64);
+    m[0] += __shfl_xor(m[0], 1, 64);
+    return m[0];
+}
+
+template <int DIR>
+__device__ inline double flux_dir(const double w[kWin], bool pos)
+{
+    if (DIR > 0)
+        return weno_up(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
+    if (DIR < 0)
+        return weno_up(w[7], w[6], w[5], w[4], w[3], w[2], w[1]);
+    return weno_up(pos ? w[0] : w[7], pos ? w[1] : w[6], pos ? w[2] : w[5], pos ? w[3] : w[4],
+                   pos ? w[4] : w[3], pos ? w[5] : w[2], pos ? w[6] : w[1]);
+}
+
+template <int NW, int SCHEME, int STAGE>
+__global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_stage(StageArgs A, int L,
+                                                                           int nchunks)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW; // teams per block
+    __shared__ double ring[TPB][8][NW][8];
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int team = wave / NW, wit = wave % NW;
+    const int h = lane >> 5;
+    const int v = wit * 32 + (lane & 31);
+    const int VP = A.VP, nz = A.nz;
+    const size_t zs = (size_t)8 * VP;
+    const double sig = h ? -1.0 : 1.0;
+
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = nb / 8, rem = nb % 8, x = b % 8, kx = b / 8;
+    const int bb = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + kx;
+    const int chunk = bb * TPB + team;
+    const bool active = chunk < nchunks;
+    const int j0 = (active ? chunk : 0) * L;
+
+    const double vi = A.vz[v];
+    const bool pos = vi > 0.0;
+    // wave-uniform upwind direction: +1 / -1 when all live bins agree
+    const bool anyp = __any(vi > 0.0), anyn = __any(vi < 0.0);
+    const int dir = anyn ? (anyp ? 0 : -1) : 1;
+    const bool store = active && v < A.nvz;
+    const double *in = A.yin + (size_t)(4 * h) * VP + v; // this lane's first field
+
+    auto zrow = [&](int m) -> int {
+        int w = m % nz;
+        return w < 0 ? w + nz : w;
+    };
+
+    double win[4][kWin];
+#pragma unroll
+    for (int kk = 0; kk < kWin; kk++) {
+        const double *r = in + (size_t)zrow(j0 - 4 + kk) * zs;
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            win[f][kk] = r[(size_t)f * VP];
+    }
+
+    const bool coll = (A.flags & kFlagColl) != 0;
+    auto moments_to_ring = [&](int kk, int zabs) {
+        const double S = win[3][kk];
+        const double R = sig * win[2][kk];
+        const double D = sig * (win[0][kk] - win[1][kk]);
+        const double m6[6] = {S, vi * S, R, vi * R, D, vi * D};
+        const double t = wave_sum6(m6, lane);
+        const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+        if ((lane & 7) == 0 && idx < 6)
+            ring[team][zabs & 7][wit][idx] = t;
+    };
+    if (coll) {
+        moments_to_ring(4, j0);
+        moments_to_ring(5, j0 + 1);
+        moments_to_ring(6, j0 + 2);
+    }
+
+    const double fac = -vi / A.dz;
+    const double fdfac = -vi / (12.0 * A.dz);
+    const double ko_eps = -A.ko / A.dz / 16.0;
+    const double mudv = A.mu * A.dv;
+    const double st = A.st * A.pmo, ct = A.ct * A.pmo;
+    const bool vac = (A.flags & kFlagVac) != 0, mat = (A.flags & kFlagMat) != 0;
+    const bool advect = !(A.flags & kFlagAdvecOff);
+
+    double Fprev[4] = {0.0, 0.0, 0.0, 0.0};
+
+    for (int j = j0 - 1; j < j0 + L; j++) {
+        double nxt[4];
+        {
+            const double *r = in + (size_t)zrow(j + 5) * zs;
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                nxt[f] = r[(size_t)f * VP];
+        }
+        const bool cell = j >= j0;
+        const size_t q = (size_t)zrow(j) * zs + (size_t)(4 * h) * VP + v;
+        double yv[4], av[4];
+        if (cell && STAGE != 0) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                yv[f] = A.y[q + (size_t)f * VP];
+                av[f] = (STAGE >= 2) ? A.acc[q + (size_t)f * VP] : 0.0;
+            }
+        }
+        if (coll)
+            moments_to_ring(7, j + 4);
+
+        double F[4] = {0.0, 0.0, 0.0, 0.0};
+        if (SCHEME == 0 && advect) {
+            if (dir > 0) {
+#pragma unroll
+                for (int f = 0; f < 4; f++)
+                    F[f] = flux_dir<1>(win[f], pos);
+            } else if (dir < 0) {
+#pragma unroll
+                for (int f = 0; f < 4; f++)
+                    F[f] = flux_dir<-1>(win[f], pos);
+            } else {
+#pragma unroll
+                for (int f = 0; f < 4; f++)
+                    F[f] = flux_dir<0>(win[f], pos);
+            }
+        }
+
+        __syncthreads();
+
+        if (cell) {
+            const double p = win[0][3], qq = win[1][3], r = win[2][3], s = win[3][3];
+            double K[4] = {0.0, 0.0, 0.0, 0.0};
+            if (advect) {
+                if (SCHEME == 0) {
+#pragma unroll
+                    for (int f = 0; f < 4; f++)
+                        K[f] = fac * (F[f] - Fprev[f]);
+                } else {
+#pragma unroll
+                    for (int f = 0; f < 4; f++) {
+                        const double *w = win[f];
+                        K[f] = fdfac * ((w[1] - w[5]) - 8.0 * (w[2] - w[4]));
+                    }
+                }
+            }
+            if (vac) {
+                K[0] += -2.0 * st * s;
+                K[1] += 2.0 * st * s;
+                K[2] += -2.0 * ct * s;
+                K[3] += 2.0 * ct * r + st * (p - qq);
+            }
+            if (mat) {
+                const double hm = sig * A.Hm[zrow(j)];
+                K[2] += hm * s;
+                K[3] -= hm * r;
+            }
+            if (coll) {
+                double M[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    const double *slot = ring[team][j & 7][w];
+#pragma unroll
+                    for (int m = 0; m < 6; m++)
+                        M[m] += slot[m];
+                }
+                const double S = mudv * (M[0] - vi * M[1]);
+                const double R = mudv * (M[2] - vi * M[3]);
+                const double D = mudv * (M[4] - vi * M[5]);
+                const double I1 = 2.0 * (r * S - sig * s * R);
+                K[0] += I1;
+                K[1] -= I1;
+                K[2] += (qq - p) * S + sig * s * D;
+                K[3] += sig * ((p - qq) * R - r * D);
+            }
+            if (SCHEME == 1) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) {
+                    const double *w = win[f];
+                    K[f] += ko_eps * ((w[1] + w[5]) - 4.0 * (w[2] + w[4]) + 6.0 * w[3]);
+                }
+            }
+            if (store) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) {
+                    const size_t o = q + (size_t)f * VP;
+                    if (STAGE == 0) {
+                        A.yout[o] = K[f];
+                    } else if (STAGE == 1) {
+                        A.acc[o] = K[f];
+                        A.yout[o] = yv[f] + A.a * K[f];
+                    } else if (STAGE == 4) {
+                        A.yout[o] = yv[f] + A.a * (K[f] + av[f]);
+                    } else {
+                        A.acc[o] = av[f] + 2.0 * K[f];
+                        A.yout[o] = yv[f] + A.a * K[f];
+                    }
+                }
+            }
+        }
+
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+#pragma unroll
+            for (int kk = 0; kk < kWin - 1; kk++)
+                win[f][kk] = win[f][kk + 1];
+            win[f][kWin - 1] = nxt[f];
+            Fprev[f] = F[f];
+        }
+    }
+}
+
+template <int NW, int SCHEME>
+hipError_t launch_nw(const StageArgs &a, int stage, int L, hipStream_t s)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
+    const int nchunks = a.nz / L;
+    dim3 grid((nchunks + TPB - 1) / TPB), block(64 * NW * TPB);
+    switch (stage) {
+    case 0: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 0>), grid, block, 0, s, a, L, nchunks); break;
+    case 1: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 1>), grid, block, 0, s, a, L, nchunks); break;
+    case 2:
+    case 3: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 2>), grid, block, 0, s, a, L, nchunks); break;
+    case 4: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 4>), grid, block, 0, s, a, L, nchunks); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int SCHEME>
+hipError_t launch_scheme2(const StageArgs &a, int stage, int L, hipStream_t s)
+{
+    switch (a.VP) {
+    case 64: return launch_nw<2, SCHEME>(a, stage, L, s);
+    case 128: return launch_nw<4, SCHEME>(a, stage, L, s);
+    case 256: return launch_nw<8, SCHEME>(a, stage, L, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 template <int W, int SCHEME>
 hipError_t launch_w(const StageArgs &a, int stage, int L, hipStream_t s)
 {
@@ -326,13 +608,24 @@ hipError_t launch_scheme(const StageArgs &a, int stage, int L, hipStream_t s)
 
 } // namespace
 
+int fast_variant()
+{
+    static int v = [] {
+        const char *e = getenv("COSENU_FAST_VARIANT");
+        return (e && *e == '1') ? 1 : 2;
+    }();
+    return v;
+}
+
 int fast_supported(int VP, int bc) { return (VP == 64 || VP == 128 || VP == 256) && bc == 0; }
 
 hipError_t launch_stage_fast(const StageArgs &a, int stage, int L, hipStream_t s)
 {
     if (!fast_supported(a.VP, a.bc) || L < 1 || a.nz % L != 0)
         return hipErrorInvalidValue;
-    return a.scheme == 0 ? launch_scheme<0>(a, stage, L, s) : launch_scheme<1>(a, stage, L, s);
+    if (fast_variant() == 1)
+        return a.scheme == 0 ? launch_scheme<0>(a, stage, L, s) : launch_scheme<1>(a, stage, L, s);
+    return a.scheme == 0 ? launch_scheme2<0>(a, stage, L, s) : launch_scheme2<1>(a, stage, L, s);
 }
 
 hipError_t launch_rhs_fast(const RhsArgs &r, int L, hipStream_t s)
@@ -350,7 +643,10 @@ const char *stage_fast_kernel_name(int VP, int stage)
 {
     static char buf[96];
     const int st = stage + 1 == 3 ? 2 : stage + 1;
-    snprintf(buf, sizeof(buf), "cosenu_fast_stage<%d, *, %d>", VP / 64, st);
+    if (fast_variant() == 1)
+        snprintf(buf, sizeof(buf), "cosenu_fast_stage<%d, *, %d>", VP / 64, st);
+    else
+        snprintf(buf, sizeof(buf), "cosenu_fast2_stage<%d, *, %d>", VP / 32, st);
     return buf;
 }
 

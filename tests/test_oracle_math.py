@@ -158,3 +158,39 @@ def test_lepton_number_and_m0_conserved_without_vacuum(oracle_lib):
     assert abs(d["lepton"] - d0["lepton"]) < 1e-12 * abs(d0["lepton"])
     assert abs(d["trace"] - d0["trace"]) < 1e-12 * d0["trace"]
     assert d["dP_max"] > 0  # the state did evolve
+
+
+def test_reference_weno_conditioning(oracle_lib):
+    """The reference's smoothness indicators are expanded quadratic forms
+    (rhs_fv.hpp:49-52) whose terms reach ~1e4 u^2 while their value is ~1e-4
+    on O(1) data with 1e-4 noise: the reference arithmetic itself then errs
+    by ~1e-11 per interface value against an extended-precision evaluation.
+    This bounds how closely any re-ordered (FMA, mirrored) evaluation can
+    track it on such data."""
+    ld = np.longdouble
+
+    def weno_ld(u):
+        u = [ld(x) for x in u]
+        r = lambda k: u[3 + k]
+        b = [r(0) * (2107 * r(0) - 9402 * r(1) + 7042 * r(2) - 1854 * r(3)) + r(1) * (11003 * r(1) - 17246 * r(2) + 4642 * r(3)) + r(2) * (7043 * r(2) - 3882 * r(3)) + 547 * r(3) * r(3),
+             r(-1) * (547 * r(-1) - 2522 * r(0) + 1922 * r(1) - 494 * r(2)) + r(0) * (3443 * r(0) - 5966 * r(1) + 1602 * r(2)) + r(1) * (2843 * r(1) - 1642 * r(2)) + 267 * r(2) * r(2),
+             r(-2) * (267 * r(-2) - 1642 * r(-1) + 1602 * r(0) - 494 * r(1)) + r(-1) * (2843 * r(-1) - 5966 * r(0) + 1922 * r(1)) + r(0) * (3443 * r(0) - 2522 * r(1)) + 547 * r(1) * r(1),
+             r(-3) * (547 * r(-3) - 3882 * r(-2) + 4642 * r(-1) - 1854 * r(0)) + r(-2) * (7043 * r(-2) - 17246 * r(-1) + 7042 * r(0)) + r(-1) * (11003 * r(-1) - 9402 * r(0)) + 2107 * r(0) * r(0)]
+        c = [ld(1) / 4 * r(0) + ld(13) / 12 * r(1) - ld(5) / 12 * r(2) + ld(1) / 12 * r(3),
+             -ld(1) / 12 * r(-1) + ld(7) / 12 * r(0) + ld(7) / 12 * r(1) - ld(1) / 12 * r(2),
+             ld(1) / 12 * r(-2) - ld(5) / 12 * r(-1) + ld(13) / 12 * r(0) + ld(1) / 4 * r(1),
+             -ld(1) / 4 * r(-3) + ld(13) / 12 * r(-2) - ld(23) / 12 * r(-1) + ld(25) / 12 * r(0)]
+        g = [ld(4), ld(18), ld(12), ld(1)]
+        w = [g[k] / (ld(1e-6) + b[k]) ** 2 for k in range(4)]
+        return sum(w[k] * c[k] for k in range(4)) / sum(w)
+
+    rng = np.random.default_rng(0)
+    noisy, smooth = [], []
+    for _ in range(500):
+        u = 1.3 + 1e-4 * rng.standard_normal(7)
+        noisy.append(abs(O.weno7(u)[0] - float(weno_ld(u))))
+        x = 0.3 + 0.01 * np.arange(-3, 4) + rng.random() * 0.01
+        u = 1.3 + 0.1 * np.sin(x)
+        smooth.append(abs(O.weno7(u)[0] - float(weno_ld(u))))
+    assert 1e-12 < max(noisy) < 1e-9     # ill-conditioned regime
+    assert max(smooth) < 1e-14           # benign regime

@@ -46,7 +46,13 @@ class _Info(ctypes.Structure):
     _fields_ = [("dz", ctypes.c_double), ("dv", ctypes.c_double), ("dt", ctypes.c_double),
                 ("ct", ctypes.c_double), ("st", ctypes.c_double), ("phy_time", ctypes.c_double),
                 ("steps_done", ctypes.c_longlong), ("device_bytes", ctypes.c_size_t),
-                ("z_chunk", ctypes.c_int)]
+                ("z_chunk", ctypes.c_int), ("fast_variant", ctypes.c_int)]
+
+
+class _BatchInfo(ctypes.Structure):
+    _fields_ = [("nruns", ctypes.c_int), ("ngroups", ctypes.c_int), ("cells", ctypes.c_longlong),
+                ("steps_done", ctypes.c_longlong), ("device_bytes", ctypes.c_size_t),
+                ("work_items", ctypes.c_longlong), ("z_chunk", ctypes.c_int)]
 
 
 class _Diag(ctypes.Structure):
@@ -96,6 +102,21 @@ def lib() -> ctypes.CDLL:
                                     ctypes.POINTER(ctypes.c_longlong)]
     L.cosenu_stage_kernel_name.argtypes = [vp, ctypes.c_int]
     L.cosenu_stage_kernel_name.restype = ctypes.c_char_p
+    L.cosenu_copy_state.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.cosenu_autotune.argtypes = [ctypes.POINTER(_Params), ip, ip, ctypes.POINTER(ctypes.c_double)]
+    L.cosenu_batch_create.argtypes = [ctypes.POINTER(_Params), ctypes.c_int, ctypes.POINTER(vp)]
+    L.cosenu_batch_destroy.argtypes = [vp]
+    L.cosenu_batch_get_info.argtypes = [vp, ctypes.POINTER(_BatchInfo)]
+    L.cosenu_batch_set_profile.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_ssize_t, ctypes.c_int]
+    L.cosenu_batch_upload_planes.argtypes = [vp, ctypes.c_int, ctypes.c_int, _planes_t,
+                                             ctypes.c_ssize_t, ctypes.c_int]
+    L.cosenu_batch_download_planes.argtypes = [vp, ctypes.c_int, ctypes.c_int, _planes_t,
+                                               ctypes.c_ssize_t, ctypes.c_int]
+    L.cosenu_batch_step.argtypes = [vp, ctypes.c_int]
+    L.cosenu_batch_sync.argtypes = [vp]
+    L.cosenu_batch_diagnostics.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Diag)]
+    L.cosenu_batch_survival_profile.argtypes = [vp, ctypes.c_int, _dp, _dp]
     _lib = L
     return L
 
@@ -236,10 +257,7 @@ class Simulation:
     def diagnostics(self) -> dict:
         d = _Diag()
         _check(lib().cosenu_diagnostics(self._h, ctypes.byref(d)), "cosenu_diagnostics")
-        return {"dP_max": d.dP_max, "dP_avg": d.dP_avg, "dbP_avg": d.dbP_avg,
-                "M0_norm": d.M0_norm, "M0_e1": d.M0[0], "M0_e2": d.M0[1], "M0_e3": d.M0[2],
-                "Pee": d.Pee, "Pbee": d.Pbee, "trace": d.trace, "btrace": d.btrace,
-                "lepton": d.lepton, "nonfinite": d.nonfinite}
+        return _diag_dict(d)
 
     def survival_profile(self):
         a = np.empty(self.p.nvz)
@@ -252,6 +270,16 @@ class Simulation:
         _check(lib().cosenu_get_info(self._h, ctypes.byref(i)), "cosenu_get_info")
         return {k: getattr(i, k) for k, _ in _Info._fields_}
 
+    def copy_state(self, src: int, dst: int):
+        _check(lib().cosenu_copy_state(self._h, src, dst), "cosenu_copy_state")
+
+    def snapshot_pee_async(self, buf: "PinnedBuffer"):
+        assert buf.n >= self.p.nvz * self.p.nz
+        _check(lib().cosenu_snapshot_pee_async(self._h, buf.ptr), "snapshot_pee_async")
+
+    def snapshot_wait(self):
+        _check(lib().cosenu_snapshot_wait(self._h), "snapshot_wait")
+
     def set_timing(self, on: bool):
         _check(lib().cosenu_set_timing(self._h, int(on)), "cosenu_set_timing")
 
@@ -263,6 +291,96 @@ class Simulation:
 
     def stage_kernel_name(self, stage: int) -> str:
         return lib().cosenu_stage_kernel_name(self._h, stage).decode()
+
+
+def _diag_dict(d: _Diag) -> dict:
+    return {"dP_max": d.dP_max, "dP_avg": d.dP_avg, "dbP_avg": d.dbP_avg,
+            "M0_norm": d.M0_norm, "M0_e1": d.M0[0], "M0_e2": d.M0[1], "M0_e3": d.M0[2],
+            "Pee": d.Pee, "Pbee": d.Pbee, "trace": d.trace, "btrace": d.btrace,
+            "lepton": d.lepton, "nonfinite": d.nonfinite}
+
+
+def autotune(p: RunParams):
+    """Time every FAST kernel variant and z chunk for p's shape; the winner is
+    used by contexts created afterwards.  Returns (variant, z_chunk, ms)."""
+    v, L, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    _check(lib().cosenu_autotune(ctypes.byref(p.c()), ctypes.byref(v), ctypes.byref(L),
+                                 ctypes.byref(ms)), "cosenu_autotune")
+    return v.value, L.value, ms.value
+
+
+class Batch:
+    """Many independent FAST-mode runs advanced together (cosenu_batch_*)."""
+
+    def __init__(self, runs: list):
+        self.runs = list(runs)
+        arr = (_Params * len(self.runs))(*[r.c() for r in self.runs])
+        h = ctypes.c_void_p()
+        _check(lib().cosenu_batch_create(arr, len(self.runs), ctypes.byref(h)), "batch_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cosenu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _planes(self, r: int, arr: np.ndarray):
+        p = self.runs[r]
+        assert arr.shape == (8, p.nvz, p.nz) and arr.dtype == np.float64 and arr.flags.c_contiguous
+        return _planes_t(*[arr[f].ctypes.data_as(_dp) for f in range(8)])
+
+    def set_profile(self, r: int, G, bG):
+        p = self.runs[r]
+        G = np.ascontiguousarray(np.broadcast_to(G, (p.nvz, p.nz)), dtype=np.float64)
+        bG = np.ascontiguousarray(np.broadcast_to(bG, (p.nvz, p.nz)), dtype=np.float64)
+        _check(lib().cosenu_batch_set_profile(self._h, r, _ptr(G), _ptr(bG), p.nz, 0),
+               "batch_set_profile")
+
+    def upload(self, r: int, state: np.ndarray, which: int = STATE):
+        s = np.ascontiguousarray(state, dtype=np.float64)
+        _check(lib().cosenu_batch_upload_planes(self._h, r, which, self._planes(r, s),
+                                                self.runs[r].nz, 0), "batch_upload")
+
+    def download(self, r: int, which: int = STATE) -> np.ndarray:
+        p = self.runs[r]
+        out = np.empty((8, p.nvz, p.nz))
+        _check(lib().cosenu_batch_download_planes(self._h, r, which, self._planes(r, out), p.nz, 0),
+               "batch_download")
+        return out
+
+    def step(self, nsteps: int = 1):
+        _check(lib().cosenu_batch_step(self._h, int(nsteps)), "batch_step")
+
+    def sync(self):
+        _check(lib().cosenu_batch_sync(self._h), "batch_sync")
+
+    def diagnostics(self, r: int) -> dict:
+        d = _Diag()
+        _check(lib().cosenu_batch_diagnostics(self._h, r, ctypes.byref(d)), "batch_diagnostics")
+        return _diag_dict(d)
+
+    def survival_profile(self, r: int):
+        p = self.runs[r]
+        a, b = np.empty(p.nvz), np.empty(p.nvz)
+        _check(lib().cosenu_batch_survival_profile(self._h, r, _ptr(a), _ptr(b)), "batch_survival")
+        return a, b
+
+    def info(self) -> dict:
+        i = _BatchInfo()
+        _check(lib().cosenu_batch_get_info(self._h, ctypes.byref(i)), "batch_get_info")
+        return {k: getattr(i, k) for k, _ in _BatchInfo._fields_}
 
 
 class PinnedBuffer:

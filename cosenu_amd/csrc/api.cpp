@@ -1,16 +1,24 @@
-// C ABI of the COSEnu MI355X library (include/cosenu_hip.h): context
-// lifetime, layout conversion on upload/download, the RK4 driver, timing,
-// diagnostics and snapshots.  Host code only; kernels live in the .hip files.
+// C ABI of the COSEnu MI355X library (include/cosenu_hip.h): context and
+// batch lifetime, layout conversion on upload/download, the RK4 drivers,
+// timing, diagnostics, snapshots and the z-chunk autotuner.  Host code only;
+// kernels live in the .hip files.
 #include "cosenu_hip.h"
 #include "common.hpp"
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <sstream>
 #include <string>
+#include <tuple>
 #include <vector>
 
 using namespace cosenu;
@@ -32,21 +40,286 @@ int fail(int code, const std::string &msg)
             return fail(COSENU_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
     } while (0)
 
+#define RC_TRY(expr)                                                                               \
+    do {                                                                                           \
+        int r_ = (expr);                                                                           \
+        if (r_)                                                                                    \
+            return r_;                                                                             \
+    } while (0)
+
 int env_int(const char *name, int dflt)
 {
     const char *s = std::getenv(name);
     return (s && *s) ? std::atoi(s) : dflt;
 }
 
+// ---------------------------------------------------------------------------
+// One state in device row layout plus what its transfers and reductions need.
+struct View {
+    double *y = nullptr, *y0 = nullptr, *prof = nullptr; // state, t=0 state, profiles
+    double *sa = nullptr, *sb = nullptr;                  // staging (stage buffers)
+    int nz = 0, nvz = 0, VP = 0;
+    size_t n_state = 0; // doubles in one state slice
+};
+
+double *which_ptr(const View &v, int which) { return which == COSENU_STATE0 ? v.y0 : v.y; }
+
+bool valid_layout(const View &v, ptrdiff_t stride, int off)
+{
+    return off >= 0 && stride >= (ptrdiff_t)off + v.nz && (size_t)stride * v.nvz <= v.n_state;
+}
+
+int view_upload_planes(const View &v, int which, const double *const planes[8], ptrdiff_t stride,
+                       int off, hipStream_t s)
+{
+    if (!planes || !valid_layout(v, stride, off) || (which != COSENU_STATE && which != COSENU_STATE0))
+        return fail(COSENU_EINVAL, "upload_planes: bad argument");
+    const size_t w = (size_t)v.nz * sizeof(double), pitch = (size_t)stride * sizeof(double);
+    for (int f = 0; f < 8; f++) {
+        if (!planes[f])
+            return fail(COSENU_EINVAL, "upload_planes: null plane");
+        double *stage = (f & 1) ? v.sb : v.sa;
+        HIP_TRY(hipMemcpy2DAsync(stage + off, pitch, planes[f] + off, pitch, w, v.nvz,
+                                 hipMemcpyHostToDevice, s));
+        HIP_TRY(launch_import_plane(which_ptr(v, which), stage, f, v.nz, v.nvz, v.VP, stride, off, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int view_download_planes(const View &v, const double *src, double *const planes[8],
+                         ptrdiff_t stride, int off, hipStream_t s)
+{
+    if (!planes || !valid_layout(v, stride, off))
+        return fail(COSENU_EINVAL, "download_planes: bad argument");
+    const size_t w = (size_t)v.nz * sizeof(double), pitch = (size_t)stride * sizeof(double);
+    for (int f = 0; f < 8; f++) {
+        if (!planes[f])
+            return fail(COSENU_EINVAL, "download_planes: null plane");
+        HIP_TRY(launch_export_plane(v.sa, src, f, v.nz, v.nvz, v.VP, stride, off, s));
+        HIP_TRY(hipMemcpy2DAsync(planes[f] + off, pitch, v.sa + off, pitch, w, v.nvz,
+                                 hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return 0;
+}
+
+int view_upload_interleaved(const View &v, int which, const double *buf, hipStream_t s)
+{
+    if (!buf || (which != COSENU_STATE && which != COSENU_STATE0))
+        return fail(COSENU_EINVAL, "upload_interleaved: bad argument");
+    const size_t n = (size_t)v.nvz * v.nz * 8;
+    HIP_TRY(hipMemcpyAsync(v.sa, buf, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_import_interleaved(which_ptr(v, which), v.sa, v.nz, v.nvz, v.VP, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int view_download_interleaved(const View &v, int which, double *buf, hipStream_t s)
+{
+    if (!buf || (which != COSENU_STATE && which != COSENU_STATE0))
+        return fail(COSENU_EINVAL, "download_interleaved: bad argument");
+    const size_t n = (size_t)v.nvz * v.nz * 8;
+    HIP_TRY(launch_export_interleaved(v.sa, which_ptr(v, which), v.nz, v.nvz, v.VP, s));
+    HIP_TRY(hipMemcpyAsync(buf, v.sa, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int view_set_profile(const View &v, const double *G, const double *bG, ptrdiff_t stride, int off,
+                     hipStream_t s)
+{
+    if (!G || !bG || !valid_layout(v, stride, off))
+        return fail(COSENU_EINVAL, "set_profile: bad argument");
+    const size_t w = (size_t)v.nz * sizeof(double), pitch = (size_t)stride * sizeof(double);
+    HIP_TRY(hipMemcpy2DAsync(v.sa + off, pitch, G + off, pitch, w, v.nvz, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpy2DAsync(v.sb + off, pitch, bG + off, pitch, w, v.nvz, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_import_profile(v.prof, v.sa, v.sb, v.nz, v.nvz, v.VP, stride, off, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int view_diagnostics(const View &v, double dz, double dv, double *part, int npart, double *dout,
+                     cosenu_diag *d, hipStream_t s)
+{
+    HIP_TRY(launch_diagnostics(v.y, v.y0, v.prof, v.nz, v.nvz, v.VP, part, npart, dout, s));
+    double h[kDiagSlots];
+    HIP_TRY(hipMemcpyAsync(h, dout, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const double dzdv = dz * dv;
+    d->dP_max = h[0] >= h[1] ? h[0] : h[1];
+    if (h[0] != h[0] || h[1] != h[1])
+        d->dP_max = NAN;
+    d->dP_avg = h[2] / h[4];
+    d->dbP_avg = h[3] / h[5];
+    d->M0[0] = h[6] * dzdv;
+    d->M0[1] = h[7] * dzdv;
+    d->M0[2] = h[8] * dzdv;
+    d->M0_norm = sqrt(d->M0[0] * d->M0[0] + d->M0[1] * d->M0[1] + d->M0[2] * d->M0[2]);
+    d->Pee = h[9] / h[11];
+    d->Pbee = h[10] / h[12];
+    d->trace = h[13] * dzdv;
+    d->btrace = h[14] * dzdv;
+    d->lepton = h[15] * dzdv;
+    d->nonfinite = h[16];
+    return 0;
+}
+
+int view_survival(const View &v, double *part, int npart, double *dout, double *pee, double *pbee,
+                  hipStream_t s)
+{
+    HIP_TRY(launch_survival_profile(v.y, v.y0, v.nz, v.nvz, v.VP, part, npart, dout, s));
+    HIP_TRY(hipMemcpyAsync(pee, dout, v.nvz * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(pbee, dout + v.VP, v.nvz * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+// Physics constants derived like NuOsc's constructor (nuosc.hpp:78-96) and
+// set_vac_pars (nuosc.hpp:132-138).
+struct Derived {
+    double dz, dv, dt, ct, st;
+};
+
+Derived derive(const cosenu_params &p)
+{
+    Derived d;
+    d.dz = (p.z1 - p.z0) / p.nz;
+    d.dv = (p.v1 - p.v0) / (p.nvz);
+    d.dt = p.CFL * d.dz;
+    double th = p.theta * M_PI / 180.0;
+    d.ct = (1.0 / 2.0) * p.omega * cos(2.0 * th);
+    d.st = (1.0 / 2.0) * p.omega * sin(2.0 * th);
+    return d;
+}
+
+int validate(const cosenu_params *p)
+{
+    if (p->scheme != COSENU_FV && p->scheme != COSENU_FD)
+        return fail(COSENU_EINVAL, "scheme must be COSENU_FV or COSENU_FD");
+    if (p->bc != COSENU_PERIODIC && p->bc != COSENU_OPEN)
+        return fail(COSENU_EINVAL, "bc must be COSENU_PERIODIC or COSENU_OPEN");
+    if (p->nvz < 1 || p->nvz > 1024)
+        return fail(COSENU_EINVAL, "nvz must be in 1..1024");
+    if (p->nz < 8)
+        return fail(COSENU_EINVAL, "nz must be at least 8");
+    if (p->flags & ~(COSENU_VAC | COSENU_MAT | COSENU_COLL | COSENU_ADVEC_OFF | COSENU_KO5))
+        return fail(COSENU_EINVAL, "unknown flag bits");
+    if ((p->flags & COSENU_KO5) && p->scheme != COSENU_FD)
+        return fail(COSENU_EINVAL, "COSENU_KO5 applies to the FD scheme only");
+    if (p->mode != COSENU_STRICT && p->mode != COSENU_FAST)
+        return fail(COSENU_EINVAL, "mode must be COSENU_STRICT or COSENU_FAST");
+    if (!(p->z1 > p->z0) || !(p->v1 > p->v0) || !(p->CFL > 0))
+        return fail(COSENU_EINVAL, "need z1 > z0, v1 > v0, CFL > 0");
+    const int VP = ((p->nvz + 63) / 64) * 64;
+    if (p->mode == COSENU_FAST) {
+        if (!fast_supported(VP, p->bc))
+            return fail(COSENU_ENOTSUP, "FAST mode supports periodic runs with nvz up to 256");
+        if ((p->flags & COSENU_KO5) != 0)
+            return fail(COSENU_ENOTSUP, "FAST mode implements the 3rd-order KO dissipation only");
+    }
+    return 0;
+}
+
+void fill_grids(const cosenu_params &p, const Derived &d, int VP, std::vector<double> &vz,
+                std::vector<double> &Hm)
+{
+    vz.assign(VP, 0.0);
+    Hm.assign(p.nz, 0.0);
+    for (int i = 0; i < p.nvz; i++)
+        vz[i] = p.v0 + (i + 0.5) * d.dv;
+    for (int j = 0; j < p.nz; j++)
+        Hm[j] = -1.0e-3 * ((p.z0 + (j + 0.5) * d.dz) + 600.0);
+}
+
+// ---------------------------------------------------------------------------
+// Autotune table: (VP, scheme, nz) -> (variant, z chunk).  Loaded from and
+// saved to the file named by COSENU_TUNE_FILE (text, one entry per line).
+struct TuneKey {
+    int VP, scheme, nz;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(VP, scheme, nz) < std::tie(o.VP, o.scheme, o.nz);
+    }
+};
+struct TuneVal {
+    int variant, chunk;
+    double ms;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+bool g_tune_loaded = false;
+
+void tune_load_locked()
+{
+    if (g_tune_loaded)
+        return;
+    g_tune_loaded = true;
+    const char *f = std::getenv("COSENU_TUNE_FILE");
+    if (!f || !*f)
+        return;
+    std::ifstream in(f);
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] == '#')
+            continue;
+        std::istringstream ss(line);
+        TuneKey k;
+        TuneVal v;
+        if (ss >> k.VP >> k.scheme >> k.nz >> v.variant >> v.chunk >> v.ms)
+            g_tune[k] = v;
+    }
+}
+
+bool tune_lookup(int VP, int scheme, int nz, TuneVal *out)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_locked();
+    auto it = g_tune.find({VP, scheme, nz});
+    if (it == g_tune.end())
+        return false;
+    *out = it->second;
+    return true;
+}
+
+void tune_store(int VP, int scheme, int nz, TuneVal v)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_locked();
+    g_tune[{VP, scheme, nz}] = v;
+    const char *f = std::getenv("COSENU_TUNE_FILE");
+    if (!f || !*f)
+        return;
+    std::ofstream out(f, std::ios::trunc);
+    out << "# VP scheme nz variant z_chunk ms_per_middle_stage\n";
+    for (auto &kv : g_tune)
+        out << kv.first.VP << " " << kv.first.scheme << " " << kv.first.nz << " "
+            << kv.second.variant << " " << kv.second.chunk << " " << kv.second.ms << "\n";
+}
+
+// Default z chunk: ~4 wavefronts per SIMD over 1024 SIMDs, within [8, 256],
+// rounded down to a divisor of nz.
+int default_chunk(int nz, int VP)
+{
+    long target = (long)nz * (VP / 64) / 4096;
+    int L = (int)std::max(8L, std::min(256L, target));
+    if (L > nz)
+        L = nz;
+    while (L > 1 && nz % L)
+        L--;
+    return L;
+}
+
 } // namespace
 
+// ===========================================================================
+// Single-run context
 struct cosenu_ctx {
     cosenu_params p;
+    Derived d;
     int nz = 0, nvz = 0, VP = 0;
-    double dz = 0, dv = 0, dt = 0, ct = 0, st = 0;
     double phy_time = 0;
     long long steps = 0;
-    size_t n_state = 0; // doubles per state in row layout
     size_t bytes = 0;
     hipStream_t stream = nullptr, copy_stream = nullptr;
     double *y = nullptr, *acc = nullptr, *A = nullptr, *B = nullptr, *y0 = nullptr;
@@ -54,14 +327,13 @@ struct cosenu_ctx {
     double *diag_part = nullptr, *diag_out = nullptr, *snap = nullptr;
     int npart = 0;
     bool have_profile = false, have_state0 = false;
-    int z_chunk = 0;
-    // timing
+    int z_chunk = 0, variant = 2;
+    View view;
     bool timing = false;
-    std::vector<hipEvent_t> ev; // start/stop pairs, stage s = (pair index) % 4
+    std::vector<hipEvent_t> ev;
     std::vector<int> ev_stage;
     double ms[4] = {0, 0, 0, 0};
     long long cnt[4] = {0, 0, 0, 0};
-    // graph of `graph_steps` RK4 steps
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     hipEvent_t snap_ready = nullptr, snap_done = nullptr;
@@ -70,15 +342,16 @@ struct cosenu_ctx {
 
 namespace {
 
-int dev_alloc(cosenu_ctx *c, double **p, size_t n)
+template <class T>
+int dev_alloc(size_t &bytes, T **p, size_t n)
 {
-    hipError_t e = hipMalloc((void **)p, n * sizeof(double));
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
     if (e != hipSuccess) {
         *p = nullptr;
-        return fail(COSENU_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(double)) +
+        return fail(COSENU_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) +
                                        " bytes failed: " + hipGetErrorString(e));
     }
-    c->bytes += n * sizeof(double);
+    bytes += n * sizeof(T);
     return 0;
 }
 
@@ -103,57 +376,35 @@ void release(cosenu_ctx *c)
         (void)hipStreamDestroy(c->copy_stream);
 }
 
-// z cells per streaming chunk in FAST mode: aim at ~4 wavefronts per SIMD
-// over 1024 SIMDs, within [8, 256], and a divisor of nz.
-int pick_chunk(int nz, int VP)
+StageArgs stage_args(const cosenu_ctx *c, int stage)
 {
-    int forced = env_int("COSENU_ZCHUNK", 0);
-    int W = VP / 64;
-    long target = (long)nz * W / 4096;
-    int L = forced > 0 ? forced : (int)std::max(8L, std::min(256L, target));
-    if (L > nz)
-        L = nz;
-    while (L > 1 && nz % L)
-        L--;
-    return L;
+    BatchRun r;
+    r.y = c->y; r.acc = c->acc; r.A = c->A; r.B = c->B;
+    r.vz = c->vz; r.Hm = c->Hm;
+    r.nz = c->nz; r.nvz = c->nvz; r.VP = c->VP;
+    r.flags = c->p.flags; r.bc = c->p.bc; r.scheme = c->p.scheme;
+    r.a[0] = 0.5 * c->d.dt; r.a[1] = 0.5 * c->d.dt; r.a[2] = c->d.dt; r.a[3] = 1.0 / 6.0 * c->d.dt;
+    r.dz = c->d.dz; r.dv = c->d.dv; r.ct = c->d.ct; r.st = c->d.st;
+    r.pmo = c->p.pmo; r.mu = c->p.mu; r.ko = c->p.ko;
+    return batch_stage_args(r, stage);
 }
 
-StageArgs stage_args(cosenu_ctx *c, int stage)
-{
-    StageArgs a;
-    a.y = c->y;
-    a.acc = c->acc;
-    switch (stage) {
-    case 1: a.yin = c->y; a.yout = c->A; a.a = 0.5 * c->dt; break;
-    case 2: a.yin = c->A; a.yout = c->B; a.a = 0.5 * c->dt; break;
-    case 3: a.yin = c->B; a.yout = c->A; a.a = c->dt; break;
-    default: a.yin = c->A; a.yout = c->y; a.a = 1.0 / 6.0 * c->dt; break;
-    }
-    a.vz = c->vz;
-    a.Hm = c->Hm;
-    a.nz = c->nz; a.nvz = c->nvz; a.VP = c->VP;
-    a.flags = c->p.flags; a.bc = c->p.bc; a.scheme = c->p.scheme;
-    a.dz = c->dz; a.dv = c->dv; a.ct = c->ct; a.st = c->st; a.pmo = c->p.pmo; a.mu = c->p.mu;
-    a.ko = c->p.ko;
-    return a;
-}
-
-RhsArgs rhs_args(cosenu_ctx *c, const double *in, double *out)
+RhsArgs rhs_args(const cosenu_ctx *c, const double *in, double *out)
 {
     RhsArgs r;
     r.in = in; r.out = out; r.vz = c->vz; r.Hm = c->Hm;
     r.nz = c->nz; r.nvz = c->nvz; r.VP = c->VP;
     r.flags = c->p.flags; r.bc = c->p.bc; r.scheme = c->p.scheme;
-    r.dz = c->dz; r.dv = c->dv; r.ct = c->ct; r.st = c->st; r.pmo = c->p.pmo; r.mu = c->p.mu;
-    r.ko = c->p.ko;
+    r.dz = c->d.dz; r.dv = c->d.dv; r.ct = c->d.ct; r.st = c->d.st;
+    r.pmo = c->p.pmo; r.mu = c->p.mu; r.ko = c->p.ko;
     return r;
 }
 
-hipError_t launch_stage(cosenu_ctx *c, int stage, hipStream_t s)
+hipError_t launch_stage(const cosenu_ctx *c, int stage, hipStream_t s)
 {
     StageArgs a = stage_args(c, stage);
     if (c->p.mode == COSENU_FAST)
-        return launch_stage_fast(a, stage, c->z_chunk, s);
+        return launch_stage_fast(a, stage, c->z_chunk, c->variant, s);
     return launch_stage_strict(a, stage, s);
 }
 
@@ -193,36 +444,26 @@ int fold_timing(cosenu_ctx *c)
     return 0;
 }
 
-int build_graph(cosenu_ctx *c, int nsteps)
+// Capture `nsteps` RK4 steps (4 launches each) on `s` into an executable graph.
+template <class Launch>
+int capture_graph(hipStream_t s, int nsteps, Launch launch_step, hipGraphExec_t *out)
 {
     hipGraph_t g = nullptr;
-    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = 0;
     for (int k = 0; k < nsteps && rc == 0; k++)
-        for (int st = 1; st <= 4 && rc == 0; st++) {
-            hipError_t e = launch_stage(c, st, c->stream);
-            if (e != hipSuccess)
-                rc = fail(COSENU_EHIP, std::string("capture launch: ") + hipGetErrorString(e));
-        }
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
+        rc = launch_step();
+    hipError_t e = hipStreamEndCapture(s, &g);
     if (rc)
         return rc;
     if (e != hipSuccess)
         return fail(COSENU_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess)
         return fail(COSENU_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    c->graph_steps = nsteps;
     return 0;
 }
-
-bool valid_layout(const cosenu_ctx *c, ptrdiff_t stride, int off)
-{
-    return off >= 0 && stride >= (ptrdiff_t)off + c->nz;
-}
-
-double *target(cosenu_ctx *c, int which) { return which == COSENU_STATE0 ? c->y0 : c->y; }
 
 } // namespace
 
@@ -236,63 +477,49 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
     if (!p || !out)
         return fail(COSENU_EINVAL, "cosenu_create: null argument");
     *out = nullptr;
-    if (p->scheme != COSENU_FV && p->scheme != COSENU_FD)
-        return fail(COSENU_EINVAL, "scheme must be COSENU_FV or COSENU_FD");
-    if (p->bc != COSENU_PERIODIC && p->bc != COSENU_OPEN)
-        return fail(COSENU_EINVAL, "bc must be COSENU_PERIODIC or COSENU_OPEN");
-    if (p->nvz < 1 || p->nvz > 1024)
-        return fail(COSENU_EINVAL, "nvz must be in 1..1024");
-    if (p->nz < 8)
-        return fail(COSENU_EINVAL, "nz must be at least 8");
-    if (p->flags & ~(COSENU_VAC | COSENU_MAT | COSENU_COLL | COSENU_ADVEC_OFF | COSENU_KO5))
-        return fail(COSENU_EINVAL, "unknown flag bits");
-    if ((p->flags & COSENU_KO5) && p->scheme != COSENU_FD)
-        return fail(COSENU_EINVAL, "COSENU_KO5 applies to the FD scheme only");
-    if (p->mode != COSENU_STRICT && p->mode != COSENU_FAST)
-        return fail(COSENU_EINVAL, "mode must be COSENU_STRICT or COSENU_FAST");
-    if (!(p->z1 > p->z0) || !(p->v1 > p->v0) || !(p->CFL > 0))
-        return fail(COSENU_EINVAL, "need z1 > z0, v1 > v0, CFL > 0");
-    const int VP = ((p->nvz + 63) / 64) * 64;
-    if (p->mode == COSENU_FAST) {
-        if (!fast_supported(VP, p->bc))
-            return fail(COSENU_ENOTSUP, "FAST mode supports periodic runs with nvz up to 256");
-        if ((p->flags & COSENU_KO5) != 0)
-            return fail(COSENU_ENOTSUP, "FAST mode implements the 3rd-order KO dissipation only");
-    }
+    RC_TRY(validate(p));
     HIP_TRY(hipSetDevice(p->device));
 
     cosenu_ctx *c = new cosenu_ctx();
     c->p = *p;
+    c->d = derive(*p);
     c->nz = p->nz;
     c->nvz = p->nvz;
-    c->VP = VP;
-    // NuOsc::NuOsc (nuosc.hpp:78-80) and set_vac_pars (nuosc.hpp:132-138)
-    c->dz = (p->z1 - p->z0) / p->nz;
-    c->dv = (p->v1 - p->v0) / (p->nvz);
-    c->dt = p->CFL * c->dz;
-    double th = p->theta * M_PI / 180.0;
-    c->ct = (1.0 / 2.0) * p->omega * cos(2.0 * th);
-    c->st = (1.0 / 2.0) * p->omega * sin(2.0 * th);
-    c->n_state = (size_t)c->nz * 8 * VP;
-    c->z_chunk = p->mode == COSENU_FAST ? pick_chunk(c->nz, VP) : 0;
+    c->VP = ((p->nvz + 63) / 64) * 64;
+    const size_t n_state = (size_t)c->nz * 8 * c->VP;
+    if (p->mode == COSENU_FAST) {
+        TuneVal tv;
+        if (tune_lookup(c->VP, p->scheme, p->nz, &tv)) {
+            c->variant = tv.variant;
+            c->z_chunk = tv.chunk;
+        } else {
+            c->variant = 2;
+            c->z_chunk = default_chunk(c->nz, c->VP);
+        }
+        int ev = env_int("COSENU_FAST_VARIANT", 0), ez = env_int("COSENU_ZCHUNK", 0);
+        if (ev == 1 || ev == 2)
+            c->variant = ev;
+        if (ez > 0 && ez <= c->nz && c->nz % ez == 0)
+            c->z_chunk = ez;
+    }
 
     int rc = 0;
     auto chk = [&](int r) {
         if (!rc)
             rc = r;
     };
-    chk(dev_alloc(c, &c->y, c->n_state));
-    if (!rc) chk(dev_alloc(c, &c->acc, c->n_state));
-    if (!rc) chk(dev_alloc(c, &c->A, c->n_state));
-    if (!rc) chk(dev_alloc(c, &c->B, c->n_state));
-    if (!rc) chk(dev_alloc(c, &c->y0, c->n_state));
-    if (!rc) chk(dev_alloc(c, &c->prof, (size_t)c->nz * 2 * VP));
-    if (!rc) chk(dev_alloc(c, &c->vz, VP));
-    if (!rc) chk(dev_alloc(c, &c->Hm, c->nz));
+    chk(dev_alloc(c->bytes, &c->y, n_state));
+    if (!rc) chk(dev_alloc(c->bytes, &c->acc, n_state));
+    if (!rc) chk(dev_alloc(c->bytes, &c->A, n_state));
+    if (!rc) chk(dev_alloc(c->bytes, &c->B, n_state));
+    if (!rc) chk(dev_alloc(c->bytes, &c->y0, n_state));
+    if (!rc) chk(dev_alloc(c->bytes, &c->prof, (size_t)c->nz * 2 * c->VP));
+    if (!rc) chk(dev_alloc(c->bytes, &c->vz, (size_t)c->VP));
+    if (!rc) chk(dev_alloc(c->bytes, &c->Hm, (size_t)c->nz));
     c->npart = diag_partials_needed(c->nz);
-    if (!rc) chk(dev_alloc(c, &c->diag_part, std::max((size_t)c->npart * kDiagSlots,
-                                                      (size_t)c->npart * 4 * VP)));
-    if (!rc) chk(dev_alloc(c, &c->diag_out, std::max(kDiagSlots, 2 * VP)));
+    if (!rc) chk(dev_alloc(c->bytes, &c->diag_part, std::max((size_t)c->npart * kDiagSlots,
+                                                             (size_t)c->npart * 4 * c->VP)));
+    if (!rc) chk(dev_alloc(c->bytes, &c->diag_out, (size_t)std::max(kDiagSlots, 2 * c->VP)));
     if (rc) {
         release(c);
         delete c;
@@ -309,17 +536,13 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
     double *bufs[] = {c->y, c->acc, c->A, c->B, c->y0};
     for (double *b : bufs)
         if (e == hipSuccess)
-            e = hipMemset(b, 0, c->n_state * sizeof(double));
+            e = hipMemset(b, 0, n_state * sizeof(double));
     if (e == hipSuccess)
-        e = hipMemset(c->prof, 0, (size_t)c->nz * 2 * VP * sizeof(double));
-    // Velocity grid and matter profile (nuosc.hpp:82-96).
-    std::vector<double> vz(VP, 0.0), Hm(c->nz);
-    for (int i = 0; i < p->nvz; i++)
-        vz[i] = p->v0 + (i + 0.5) * c->dv;
-    for (int j = 0; j < c->nz; j++)
-        Hm[j] = -1.0e-3 * ((p->z0 + (j + 0.5) * c->dz) + 600.0);
+        e = hipMemset(c->prof, 0, (size_t)c->nz * 2 * c->VP * sizeof(double));
+    std::vector<double> vz, Hm;
+    fill_grids(*p, c->d, c->VP, vz, Hm);
     if (e == hipSuccess)
-        e = hipMemcpy(c->vz, vz.data(), VP * sizeof(double), hipMemcpyHostToDevice);
+        e = hipMemcpy(c->vz, vz.data(), c->VP * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(c->Hm, Hm.data(), c->nz * sizeof(double), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -327,6 +550,15 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
         delete c;
         return fail(COSENU_EHIP, std::string("cosenu_create: ") + hipGetErrorString(e));
     }
+    c->view.y = c->y;
+    c->view.y0 = c->y0;
+    c->view.prof = c->prof;
+    c->view.sa = c->A;
+    c->view.sb = c->B;
+    c->view.nz = c->nz;
+    c->view.nvz = c->nvz;
+    c->view.VP = c->VP;
+    c->view.n_state = n_state;
     *out = c;
     return 0;
 }
@@ -347,34 +579,26 @@ int cosenu_get_info(cosenu_ctx *c, cosenu_info *info)
 {
     if (!c || !info)
         return fail(COSENU_EINVAL, "cosenu_get_info: null argument");
-    info->dz = c->dz;
-    info->dv = c->dv;
-    info->dt = c->dt;
-    info->ct = c->ct;
-    info->st = c->st;
+    info->dz = c->d.dz;
+    info->dv = c->d.dv;
+    info->dt = c->d.dt;
+    info->ct = c->d.ct;
+    info->st = c->d.st;
     info->phy_time = c->phy_time;
     info->steps_done = c->steps;
     info->device_bytes = c->bytes;
     info->z_chunk = c->z_chunk;
+    info->fast_variant = c->p.mode == COSENU_FAST ? c->variant : 0;
     return 0;
 }
 
 int cosenu_set_profile(cosenu_ctx *c, const double *G, const double *bG, ptrdiff_t stride, int off)
 {
-    if (!c || !G || !bG || !valid_layout(c, stride, off))
-        return fail(COSENU_EINVAL, "cosenu_set_profile: bad argument");
+    if (!c)
+        return fail(COSENU_EINVAL, "cosenu_set_profile: null context");
     HIP_TRY(hipSetDevice(c->p.device));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    // stage the two planes in A and B
-    const size_t w = (size_t)c->nz * sizeof(double), pitch = (size_t)stride * sizeof(double);
-    if ((size_t)stride * c->nvz > c->n_state)
-        return fail(COSENU_EINVAL, "cosenu_set_profile: row_stride too large");
-    HIP_TRY(hipMemcpy2DAsync(c->A + off, pitch, G + off, pitch, w, c->nvz, hipMemcpyHostToDevice,
-                             c->stream));
-    HIP_TRY(hipMemcpy2DAsync(c->B + off, pitch, bG + off, pitch, w, c->nvz, hipMemcpyHostToDevice,
-                             c->stream));
-    HIP_TRY(launch_import_profile(c->prof, c->A, c->B, c->nz, c->nvz, c->VP, stride, off, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    RC_TRY(view_set_profile(c->view, G, bG, stride, off, c->stream));
     c->have_profile = true;
     return 0;
 }
@@ -382,23 +606,11 @@ int cosenu_set_profile(cosenu_ctx *c, const double *G, const double *bG, ptrdiff
 int cosenu_upload_planes(cosenu_ctx *c, int which, const double *const planes[8], ptrdiff_t stride,
                          int off)
 {
-    if (!c || !planes || !valid_layout(c, stride, off) || (which != COSENU_STATE && which != COSENU_STATE0))
-        return fail(COSENU_EINVAL, "cosenu_upload_planes: bad argument");
-    if ((size_t)stride * c->nvz > c->n_state)
-        return fail(COSENU_EINVAL, "cosenu_upload_planes: row_stride too large");
+    if (!c)
+        return fail(COSENU_EINVAL, "cosenu_upload_planes: null context");
     HIP_TRY(hipSetDevice(c->p.device));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    double *dst = target(c, which);
-    const size_t w = (size_t)c->nz * sizeof(double), pitch = (size_t)stride * sizeof(double);
-    for (int f = 0; f < 8; f++) {
-        if (!planes[f])
-            return fail(COSENU_EINVAL, "cosenu_upload_planes: null plane");
-        double *stage = (f & 1) ? c->B : c->A;
-        HIP_TRY(hipMemcpy2DAsync(stage + off, pitch, planes[f] + off, pitch, w, c->nvz,
-                                 hipMemcpyHostToDevice, c->stream));
-        HIP_TRY(launch_import_plane(dst, stage, f, c->nz, c->nvz, c->VP, stride, off, c->stream));
-    }
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    RC_TRY(view_upload_planes(c->view, which, planes, stride, off, c->stream));
     if (which == COSENU_STATE0)
         c->have_state0 = true;
     return 0;
@@ -407,34 +619,19 @@ int cosenu_upload_planes(cosenu_ctx *c, int which, const double *const planes[8]
 int cosenu_download_planes(cosenu_ctx *c, int which, double *const planes[8], ptrdiff_t stride,
                            int off)
 {
-    if (!c || !planes || !valid_layout(c, stride, off) || (which != COSENU_STATE && which != COSENU_STATE0))
+    if (!c || (which != COSENU_STATE && which != COSENU_STATE0))
         return fail(COSENU_EINVAL, "cosenu_download_planes: bad argument");
-    if ((size_t)stride * c->nvz > c->n_state)
-        return fail(COSENU_EINVAL, "cosenu_download_planes: row_stride too large");
     HIP_TRY(hipSetDevice(c->p.device));
-    const double *src = target(c, which);
-    const size_t w = (size_t)c->nz * sizeof(double), pitch = (size_t)stride * sizeof(double);
-    for (int f = 0; f < 8; f++) {
-        if (!planes[f])
-            return fail(COSENU_EINVAL, "cosenu_download_planes: null plane");
-        HIP_TRY(launch_export_plane(c->A, src, f, c->nz, c->nvz, c->VP, stride, off, c->stream));
-        HIP_TRY(hipMemcpy2DAsync(planes[f] + off, pitch, c->A + off, pitch, w, c->nvz,
-                                 hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-    }
-    return 0;
+    return view_download_planes(c->view, which_ptr(c->view, which), planes, stride, off, c->stream);
 }
 
 int cosenu_upload_interleaved(cosenu_ctx *c, int which, const double *buf)
 {
-    if (!c || !buf || (which != COSENU_STATE && which != COSENU_STATE0))
-        return fail(COSENU_EINVAL, "cosenu_upload_interleaved: bad argument");
+    if (!c)
+        return fail(COSENU_EINVAL, "cosenu_upload_interleaved: null context");
     HIP_TRY(hipSetDevice(c->p.device));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    const size_t n = (size_t)c->nvz * c->nz * 8;
-    HIP_TRY(hipMemcpyAsync(c->A, buf, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(launch_import_interleaved(target(c, which), c->A, c->nz, c->nvz, c->VP, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    RC_TRY(view_upload_interleaved(c->view, which, buf, c->stream));
     if (which == COSENU_STATE0)
         c->have_state0 = true;
     return 0;
@@ -442,13 +639,24 @@ int cosenu_upload_interleaved(cosenu_ctx *c, int which, const double *buf)
 
 int cosenu_download_interleaved(cosenu_ctx *c, int which, double *buf)
 {
-    if (!c || !buf || (which != COSENU_STATE && which != COSENU_STATE0))
-        return fail(COSENU_EINVAL, "cosenu_download_interleaved: bad argument");
+    if (!c)
+        return fail(COSENU_EINVAL, "cosenu_download_interleaved: null context");
     HIP_TRY(hipSetDevice(c->p.device));
-    const size_t n = (size_t)c->nvz * c->nz * 8;
-    HIP_TRY(launch_export_interleaved(c->A, target(c, which), c->nz, c->nvz, c->VP, c->stream));
-    HIP_TRY(hipMemcpyAsync(buf, c->A, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return view_download_interleaved(c->view, which, buf, c->stream);
+}
+
+int cosenu_copy_state(cosenu_ctx *c, int from, int to)
+{
+    if (!c || (from != COSENU_STATE && from != COSENU_STATE0) ||
+        (to != COSENU_STATE && to != COSENU_STATE0))
+        return fail(COSENU_EINVAL, "cosenu_copy_state: bad argument");
+    HIP_TRY(hipSetDevice(c->p.device));
+    if (from != to)
+        HIP_TRY(hipMemcpyAsync(which_ptr(c->view, to), which_ptr(c->view, from),
+                               c->view.n_state * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (to == COSENU_STATE0)
+        c->have_state0 = true;
     return 0;
 }
 
@@ -465,26 +673,28 @@ int cosenu_step(cosenu_ctx *c, int nsteps)
                 (void)hipGraphExecDestroy(c->graph);
                 c->graph = nullptr;
             }
-            int rc = build_graph(c, gsteps);
-            if (rc)
-                return rc;
+            auto one = [c]() -> int {
+                for (int st = 1; st <= 4; st++) {
+                    hipError_t e = launch_stage(c, st, c->stream);
+                    if (e != hipSuccess)
+                        return fail(COSENU_EHIP, std::string("capture launch: ") + hipGetErrorString(e));
+                }
+                return 0;
+            };
+            RC_TRY(capture_graph(c->stream, gsteps, one, &c->graph));
+            c->graph_steps = gsteps;
         }
         for (; done + gsteps <= nsteps; done += gsteps)
             HIP_TRY(hipGraphLaunch(c->graph, c->stream));
     }
     for (; done < nsteps; done++) {
-        int rc = enqueue_step(c, c->stream);
-        if (rc)
-            return rc;
-        if (c->timing && c->ev.size() > 8192) {
-            rc = fold_timing(c);
-            if (rc)
-                return rc;
-        }
+        RC_TRY(enqueue_step(c, c->stream));
+        if (c->timing && c->ev.size() > 8192)
+            RC_TRY(fold_timing(c));
     }
     // phy_time += dt per step, as step_rk4 does (nuosc.hpp:347)
     for (int k = 0; k < nsteps; k++)
-        c->phy_time += c->dt;
+        c->phy_time += c->d.dt;
     c->steps += nsteps;
     return 0;
 }
@@ -500,24 +710,17 @@ int cosenu_sync(cosenu_ctx *c)
 
 int cosenu_rhs(cosenu_ctx *c, double *const out[8], ptrdiff_t stride, int off)
 {
-    if (!c || !out || !valid_layout(c, stride, off))
-        return fail(COSENU_EINVAL, "cosenu_rhs: bad argument");
-    if ((size_t)stride * c->nvz > c->n_state)
-        return fail(COSENU_EINVAL, "cosenu_rhs: row_stride too large");
+    if (!c)
+        return fail(COSENU_EINVAL, "cosenu_rhs: null context");
+    if (!valid_layout(c->view, stride, off))
+        return fail(COSENU_EINVAL, "cosenu_rhs: bad layout");
     HIP_TRY(hipSetDevice(c->p.device));
     RhsArgs r = rhs_args(c, c->y, c->B);
     if (c->p.mode == COSENU_FAST)
-        HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->stream));
+        HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->variant, c->stream));
     else
         HIP_TRY(launch_rhs_strict(r, c->stream));
-    const size_t w = (size_t)c->nz * sizeof(double), pitch = (size_t)stride * sizeof(double);
-    for (int f = 0; f < 8; f++) {
-        HIP_TRY(launch_export_plane(c->A, c->B, f, c->nz, c->nvz, c->VP, stride, off, c->stream));
-        HIP_TRY(hipMemcpy2DAsync(out[f] + off, pitch, c->A + off, pitch, w, c->nvz,
-                                 hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-    }
-    return 0;
+    return view_download_planes(c->view, c->B, out, stride, off, c->stream);
 }
 
 int cosenu_diagnostics(cosenu_ctx *c, cosenu_diag *d)
@@ -527,28 +730,8 @@ int cosenu_diagnostics(cosenu_ctx *c, cosenu_diag *d)
     if (!c->have_profile || !c->have_state0)
         return fail(COSENU_ESTATE, "cosenu_diagnostics: set the profile and the t=0 state first");
     HIP_TRY(hipSetDevice(c->p.device));
-    HIP_TRY(launch_diagnostics(c->y, c->y0, c->prof, c->nz, c->nvz, c->VP, c->diag_part, c->npart,
-                               c->diag_out, c->stream));
-    double h[kDiagSlots];
-    HIP_TRY(hipMemcpyAsync(h, c->diag_out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    const double dzdv = c->dz * c->dv;
-    d->dP_max = h[0] >= h[1] ? h[0] : h[1];
-    if (h[0] != h[0] || h[1] != h[1])
-        d->dP_max = NAN;
-    d->dP_avg = h[2] / h[4];
-    d->dbP_avg = h[3] / h[5];
-    d->M0[0] = h[6] * dzdv;
-    d->M0[1] = h[7] * dzdv;
-    d->M0[2] = h[8] * dzdv;
-    d->M0_norm = sqrt(d->M0[0] * d->M0[0] + d->M0[1] * d->M0[1] + d->M0[2] * d->M0[2]);
-    d->Pee = h[9] / h[11];
-    d->Pbee = h[10] / h[12];
-    d->trace = h[13] * dzdv;
-    d->btrace = h[14] * dzdv;
-    d->lepton = h[15] * dzdv;
-    d->nonfinite = h[16];
-    return 0;
+    return view_diagnostics(c->view, c->d.dz, c->d.dv, c->diag_part, c->npart, c->diag_out, d,
+                            c->stream);
 }
 
 int cosenu_survival_profile(cosenu_ctx *c, double *pee, double *pbee)
@@ -558,14 +741,7 @@ int cosenu_survival_profile(cosenu_ctx *c, double *pee, double *pbee)
     if (!c->have_state0)
         return fail(COSENU_ESTATE, "cosenu_survival_profile: set the t=0 state first");
     HIP_TRY(hipSetDevice(c->p.device));
-    HIP_TRY(launch_survival_profile(c->y, c->y0, c->nz, c->nvz, c->VP, c->diag_part, c->npart,
-                                    c->diag_out, c->stream));
-    HIP_TRY(hipMemcpyAsync(pee, c->diag_out, c->nvz * sizeof(double), hipMemcpyDeviceToHost,
-                           c->stream));
-    HIP_TRY(hipMemcpyAsync(pbee, c->diag_out + c->VP, c->nvz * sizeof(double),
-                           hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return 0;
+    return view_survival(c->view, c->diag_part, c->npart, c->diag_out, pee, pbee, c->stream);
 }
 
 void *cosenu_host_alloc(size_t bytes)
@@ -592,11 +768,8 @@ int cosenu_snapshot_pee_async(cosenu_ctx *c, double *host)
         return fail(COSENU_ESTATE, "cosenu_snapshot_pee_async: set the t=0 state first");
     HIP_TRY(hipSetDevice(c->p.device));
     const size_t n = (size_t)c->nvz * c->nz;
-    if (!c->snap) {
-        int rc = dev_alloc(c, &c->snap, n);
-        if (rc)
-            return rc;
-    }
+    if (!c->snap)
+        RC_TRY(dev_alloc(c->bytes, &c->snap, n));
     // the previous copy must have drained before the buffer is rewritten
     HIP_TRY(hipStreamWaitEvent(c->stream, c->snap_done, 0));
     HIP_TRY(launch_pee_plane(c->snap, c->y, c->y0, c->nz, c->nvz, c->VP, c->stream));
@@ -624,9 +797,7 @@ int cosenu_set_timing(cosenu_ctx *c, int enable)
     if (!c)
         return fail(COSENU_EINVAL, "cosenu_set_timing: null context");
     HIP_TRY(hipSetDevice(c->p.device));
-    int rc = fold_timing(c);
-    if (rc)
-        return rc;
+    RC_TRY(fold_timing(c));
     for (int s = 0; s < 4; s++) {
         c->ms[s] = 0;
         c->cnt[s] = 0;
@@ -640,9 +811,7 @@ int cosenu_get_timing(cosenu_ctx *c, double ms[4], long long counts[4])
     if (!c || !ms || !counts)
         return fail(COSENU_EINVAL, "cosenu_get_timing: null argument");
     HIP_TRY(hipSetDevice(c->p.device));
-    int rc = fold_timing(c);
-    if (rc)
-        return rc;
+    RC_TRY(fold_timing(c));
     for (int s = 0; s < 4; s++) {
         ms[s] = c->ms[s];
         counts[s] = c->cnt[s];
@@ -654,8 +823,425 @@ const char *cosenu_stage_kernel_name(cosenu_ctx *c, int stage)
 {
     if (!c)
         return "";
-    return c->p.mode == COSENU_FAST ? stage_fast_kernel_name(c->VP, stage)
+    return c->p.mode == COSENU_FAST ? stage_fast_kernel_name(c->VP, stage, c->variant)
                                     : stage_strict_kernel_name(stage);
+}
+
+// Times the middle RK stage (stages 2 and 3 share its kernel) for every
+// kernel variant and z chunk that divides nz, on a scratch context of this
+// shape, and records the fastest in the tune table.
+int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *ms_best)
+{
+    if (!p)
+        return fail(COSENU_EINVAL, "cosenu_autotune: null argument");
+    cosenu_params q = *p;
+    q.mode = COSENU_FAST;
+    RC_TRY(validate(&q));
+    cosenu_ctx *c = nullptr;
+    RC_TRY(cosenu_create(&q, &c));
+    int rc = 0;
+    double best = 1e300;
+    int bv = 2, bl = c->z_chunk;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = fail(COSENU_EHIP, "cosenu_autotune: hipEventCreate failed");
+    const int cands[] = {8, 16, 32, 64, 128, 256};
+    for (int var = 1; var <= 2 && !rc; var++)
+        for (int L : cands) {
+            if (rc || L > c->nz || c->nz % L)
+                continue;
+            c->variant = var;
+            c->z_chunk = L;
+            const int reps = 5;
+            for (int k = 0; k < 2 + reps && !rc; k++) {
+                if (k == 2 && hipEventRecord(e0, c->stream) != hipSuccess)
+                    rc = fail(COSENU_EHIP, "cosenu_autotune: event record");
+                if (!rc && launch_stage(c, 2, c->stream) != hipSuccess)
+                    rc = fail(COSENU_EHIP, "cosenu_autotune: launch failed");
+            }
+            if (rc)
+                break;
+            float ms = 0;
+            if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = fail(COSENU_EHIP, "cosenu_autotune: timing failed");
+                break;
+            }
+            ms /= reps;
+            if (ms < best) {
+                best = ms;
+                bv = var;
+                bl = L;
+            }
+        }
+    if (e0)
+        (void)hipEventDestroy(e0);
+    if (e1)
+        (void)hipEventDestroy(e1);
+    cosenu_destroy(c);
+    if (rc)
+        return rc;
+    tune_store(((q.nvz + 63) / 64) * 64, q.scheme, q.nz, {bv, bl, best});
+    if (variant)
+        *variant = bv;
+    if (z_chunk)
+        *z_chunk = bl;
+    if (ms_best)
+        *ms_best = best;
+    return 0;
+}
+
+} // extern "C"
+
+// ===========================================================================
+// Batched runs
+namespace {
+
+struct Group {
+    int VP = 0, scheme = 0, flags = 0, L = 0;
+    std::vector<int> runs;       // batch run indices
+    size_t n_state = 0, n_prof = 0, n_vz = 0, n_hm = 0;
+    double *y = nullptr, *acc = nullptr, *A = nullptr, *B = nullptr, *y0 = nullptr;
+    double *prof = nullptr, *vz = nullptr, *Hm = nullptr;
+    BatchRun *d_runs = nullptr;
+    int2 *d_items = nullptr;
+    int nitems = 0;
+};
+
+} // namespace
+
+struct cosenu_batch {
+    int nruns = 0, device = 0;
+    std::vector<cosenu_params> P;
+    std::vector<Derived> D;
+    std::vector<int> run_group;  // group of each run
+    std::vector<int> run_slot;   // index within the group
+    std::vector<size_t> off_state, off_prof; // offsets inside the group buffers
+    std::vector<View> views;
+    std::vector<bool> have_profile, have_state0;
+    std::vector<Group> groups;
+    hipStream_t stream = nullptr;
+    double *diag_part = nullptr, *diag_out = nullptr;
+    int npart = 1024;
+    size_t bytes = 0;
+    long long cells = 0, steps = 0;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+};
+
+namespace {
+
+void release_batch(cosenu_batch *b)
+{
+    for (Group &g : b->groups) {
+        double *bufs[] = {g.y, g.acc, g.A, g.B, g.y0, g.prof, g.vz, g.Hm};
+        for (double *p : bufs)
+            if (p)
+                (void)hipFree(p);
+        if (g.d_runs)
+            (void)hipFree(g.d_runs);
+        if (g.d_items)
+            (void)hipFree(g.d_items);
+    }
+    if (b->diag_part)
+        (void)hipFree(b->diag_part);
+    if (b->diag_out)
+        (void)hipFree(b->diag_out);
+    if (b->graph)
+        (void)hipGraphExecDestroy(b->graph);
+    if (b->stream)
+        (void)hipStreamDestroy(b->stream);
+}
+
+int batch_enqueue_step(cosenu_batch *b, hipStream_t s)
+{
+    for (int st = 1; st <= 4; st++)
+        for (Group &g : b->groups)
+            HIP_TRY(launch_stage_fast_batch(g.d_runs, g.d_items, g.nitems, g.VP, g.scheme, g.L, st, s));
+    return 0;
+}
+
+// Largest tuned (or default) chunk that divides every nz of the group.
+int group_chunk(const cosenu_batch *b, const Group &g)
+{
+    int gcd = 0;
+    for (int r : g.runs)
+        gcd = std::gcd(gcd, b->P[r].nz);
+    int want = env_int("COSENU_BATCH_ZCHUNK", 0);
+    if (want <= 0) {
+        TuneVal tv;
+        want = tune_lookup(g.VP, g.scheme, -1, &tv) ? tv.chunk : 32;
+    }
+    int L = std::min(want, gcd);
+    while (L > 1 && gcd % L)
+        L--;
+    return std::max(L, 1);
+}
+
+} // namespace
+
+extern "C" {
+
+int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out)
+{
+    if (!runs || nruns < 1 || !out)
+        return fail(COSENU_EINVAL, "cosenu_batch_create: bad argument");
+    *out = nullptr;
+    for (int r = 0; r < nruns; r++) {
+        cosenu_params q = runs[r];
+        q.mode = COSENU_FAST;
+        int rc = validate(&q);
+        if (rc)
+            return fail(rc, "run " + std::to_string(r) + ": " + g_err);
+        if (runs[r].device != runs[0].device)
+            return fail(COSENU_EINVAL, "all runs of a batch must use one device");
+    }
+    HIP_TRY(hipSetDevice(runs[0].device));
+    cosenu_batch *b = new cosenu_batch();
+    b->nruns = nruns;
+    b->device = runs[0].device;
+    b->P.assign(runs, runs + nruns);
+    b->run_group.resize(nruns);
+    b->run_slot.resize(nruns);
+    b->off_state.resize(nruns);
+    b->off_prof.resize(nruns);
+    b->views.resize(nruns);
+    b->have_profile.assign(nruns, false);
+    b->have_state0.assign(nruns, false);
+    // group by (VP, scheme)
+    std::map<std::pair<int, int>, int> gid;
+    for (int r = 0; r < nruns; r++) {
+        b->P[r].mode = COSENU_FAST;
+        b->D.push_back(derive(b->P[r]));
+        const int VP = ((runs[r].nvz + 63) / 64) * 64;
+        auto key = std::make_pair(VP, runs[r].scheme);
+        if (!gid.count(key)) {
+            gid[key] = (int)b->groups.size();
+            Group g;
+            g.VP = VP;
+            g.scheme = runs[r].scheme;
+            b->groups.push_back(g);
+        }
+        Group &g = b->groups[gid[key]];
+        b->run_group[r] = gid[key];
+        b->run_slot[r] = (int)g.runs.size();
+        b->off_state[r] = g.n_state;
+        b->off_prof[r] = g.n_prof;
+        g.runs.push_back(r);
+        g.n_state += (size_t)runs[r].nz * 8 * VP;
+        g.n_prof += (size_t)runs[r].nz * 2 * VP;
+        g.n_vz += VP;
+        g.n_hm += runs[r].nz;
+        b->cells += (long long)runs[r].nz * runs[r].nvz;
+    }
+    int rc = 0;
+    for (Group &g : b->groups) {
+        if (!rc) rc = dev_alloc(b->bytes, &g.y, g.n_state);
+        if (!rc) rc = dev_alloc(b->bytes, &g.acc, g.n_state);
+        if (!rc) rc = dev_alloc(b->bytes, &g.A, g.n_state);
+        if (!rc) rc = dev_alloc(b->bytes, &g.B, g.n_state);
+        if (!rc) rc = dev_alloc(b->bytes, &g.y0, g.n_state);
+        if (!rc) rc = dev_alloc(b->bytes, &g.prof, g.n_prof);
+        if (!rc) rc = dev_alloc(b->bytes, &g.vz, g.n_vz);
+        if (!rc) rc = dev_alloc(b->bytes, &g.Hm, g.n_hm);
+    }
+    int maxVP = 64;
+    for (Group &g : b->groups)
+        maxVP = std::max(maxVP, g.VP);
+    if (!rc) rc = dev_alloc(b->bytes, &b->diag_part, std::max((size_t)b->npart * kDiagSlots,
+                                                              (size_t)b->npart * 4 * maxVP));
+    if (!rc) rc = dev_alloc(b->bytes, &b->diag_out, (size_t)std::max(kDiagSlots, 2 * maxVP));
+    hipError_t e = hipSuccess;
+    if (!rc)
+        e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    for (Group &g : b->groups) {
+        if (rc || e != hipSuccess)
+            break;
+        double *bufs[] = {g.y, g.acc, g.A, g.B, g.y0};
+        for (double *p : bufs)
+            if (e == hipSuccess)
+                e = hipMemset(p, 0, g.n_state * sizeof(double));
+        if (e == hipSuccess)
+            e = hipMemset(g.prof, 0, g.n_prof * sizeof(double));
+        // per-run tables
+        g.L = group_chunk(b, g);
+        std::vector<BatchRun> hr;
+        std::vector<double> vz_all, hm_all;
+        std::vector<int2> items;
+        size_t vo = 0, ho = 0;
+        for (size_t k = 0; k < g.runs.size(); k++) {
+            const int r = g.runs[k];
+            const cosenu_params &p = b->P[r];
+            const Derived &d = b->D[r];
+            std::vector<double> vz, Hm;
+            fill_grids(p, d, g.VP, vz, Hm);
+            vz_all.insert(vz_all.end(), vz.begin(), vz.end());
+            hm_all.insert(hm_all.end(), Hm.begin(), Hm.end());
+            BatchRun br;
+            const size_t so = b->off_state[r];
+            br.y = g.y + so; br.acc = g.acc + so; br.A = g.A + so; br.B = g.B + so;
+            br.vz = g.vz + vo; br.Hm = g.Hm + ho;
+            br.nz = p.nz; br.nvz = p.nvz; br.VP = g.VP;
+            br.flags = p.flags; br.bc = p.bc; br.scheme = p.scheme;
+            br.a[0] = 0.5 * d.dt; br.a[1] = 0.5 * d.dt; br.a[2] = d.dt; br.a[3] = 1.0 / 6.0 * d.dt;
+            br.dz = d.dz; br.dv = d.dv; br.ct = d.ct; br.st = d.st;
+            br.pmo = p.pmo; br.mu = p.mu; br.ko = p.ko;
+            hr.push_back(br);
+            for (int j0 = 0; j0 < p.nz; j0 += g.L)
+                items.push_back(make_int2((int)k, j0));
+            View &v = b->views[r];
+            v.y = br.y; v.y0 = g.y0 + so; v.prof = g.prof + b->off_prof[r];
+            v.sa = br.A; v.sb = br.B;
+            v.nz = p.nz; v.nvz = p.nvz; v.VP = g.VP;
+            v.n_state = (size_t)p.nz * 8 * g.VP;
+            vo += g.VP;
+            ho += p.nz;
+        }
+        g.nitems = (int)items.size();
+        if (e == hipSuccess) e = hipMalloc((void **)&g.d_runs, hr.size() * sizeof(BatchRun));
+        if (e == hipSuccess) e = hipMalloc((void **)&g.d_items, items.size() * sizeof(int2));
+        if (e == hipSuccess)
+            e = hipMemcpy(g.d_runs, hr.data(), hr.size() * sizeof(BatchRun), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(g.d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(g.vz, vz_all.data(), vz_all.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(g.Hm, hm_all.data(), hm_all.size() * sizeof(double), hipMemcpyHostToDevice);
+    }
+    if (!rc && e != hipSuccess)
+        rc = fail(COSENU_EHIP, std::string("cosenu_batch_create: ") + hipGetErrorString(e));
+    if (rc) {
+        release_batch(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return 0;
+}
+
+int cosenu_batch_destroy(cosenu_batch *b)
+{
+    if (!b)
+        return 0;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    release_batch(b);
+    delete b;
+    return 0;
+}
+
+static int batch_run_ok(cosenu_batch *b, int run)
+{
+    if (!b || run < 0 || run >= b->nruns)
+        return fail(COSENU_EINVAL, "batch: bad run index");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int cosenu_batch_set_profile(cosenu_batch *b, int run, const double *G, const double *bG,
+                             ptrdiff_t stride, int off)
+{
+    RC_TRY(batch_run_ok(b, run));
+    RC_TRY(view_set_profile(b->views[run], G, bG, stride, off, b->stream));
+    b->have_profile[run] = true;
+    return 0;
+}
+
+int cosenu_batch_upload_planes(cosenu_batch *b, int run, int which, const double *const planes[8],
+                               ptrdiff_t stride, int off)
+{
+    RC_TRY(batch_run_ok(b, run));
+    RC_TRY(view_upload_planes(b->views[run], which, planes, stride, off, b->stream));
+    if (which == COSENU_STATE0)
+        b->have_state0[run] = true;
+    return 0;
+}
+
+int cosenu_batch_download_planes(cosenu_batch *b, int run, int which, double *const planes[8],
+                                 ptrdiff_t stride, int off)
+{
+    RC_TRY(batch_run_ok(b, run));
+    if (which != COSENU_STATE && which != COSENU_STATE0)
+        return fail(COSENU_EINVAL, "cosenu_batch_download_planes: bad `which`");
+    return view_download_planes(b->views[run], which_ptr(b->views[run], which), planes, stride, off,
+                                b->stream);
+}
+
+int cosenu_batch_step(cosenu_batch *b, int nsteps)
+{
+    if (!b || nsteps < 0)
+        return fail(COSENU_EINVAL, "cosenu_batch_step: bad argument");
+    HIP_TRY(hipSetDevice(b->device));
+    const int gsteps = env_int("COSENU_GRAPH_STEPS", 16);
+    int done = 0;
+    if (gsteps > 0 && nsteps >= gsteps) {
+        if (!b->graph || b->graph_steps != gsteps) {
+            if (b->graph) {
+                (void)hipGraphExecDestroy(b->graph);
+                b->graph = nullptr;
+            }
+            auto one = [b]() -> int { return batch_enqueue_step(b, b->stream); };
+            RC_TRY(capture_graph(b->stream, gsteps, one, &b->graph));
+            b->graph_steps = gsteps;
+        }
+        for (; done + gsteps <= nsteps; done += gsteps)
+            HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+    }
+    for (; done < nsteps; done++)
+        RC_TRY(batch_enqueue_step(b, b->stream));
+    b->steps += nsteps;
+    return 0;
+}
+
+int cosenu_batch_sync(cosenu_batch *b)
+{
+    if (!b)
+        return fail(COSENU_EINVAL, "cosenu_batch_sync: null batch");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int cosenu_batch_diagnostics(cosenu_batch *b, int run, cosenu_diag *d)
+{
+    RC_TRY(batch_run_ok(b, run));
+    if (!d)
+        return fail(COSENU_EINVAL, "cosenu_batch_diagnostics: null argument");
+    if (!b->have_profile[run] || !b->have_state0[run])
+        return fail(COSENU_ESTATE, "cosenu_batch_diagnostics: set the profile and the t=0 state first");
+    return view_diagnostics(b->views[run], b->D[run].dz, b->D[run].dv, b->diag_part,
+                            std::min(b->npart, diag_partials_needed(b->P[run].nz)), b->diag_out, d,
+                            b->stream);
+}
+
+int cosenu_batch_survival_profile(cosenu_batch *b, int run, double *pee, double *pbee)
+{
+    RC_TRY(batch_run_ok(b, run));
+    if (!pee || !pbee)
+        return fail(COSENU_EINVAL, "cosenu_batch_survival_profile: null argument");
+    if (!b->have_state0[run])
+        return fail(COSENU_ESTATE, "cosenu_batch_survival_profile: set the t=0 state first");
+    return view_survival(b->views[run], b->diag_part,
+                         std::min(b->npart, diag_partials_needed(b->P[run].nz)), b->diag_out, pee,
+                         pbee, b->stream);
+}
+
+int cosenu_batch_get_info(cosenu_batch *b, cosenu_batch_info *info)
+{
+    if (!b || !info)
+        return fail(COSENU_EINVAL, "cosenu_batch_get_info: null argument");
+    info->nruns = b->nruns;
+    info->ngroups = (int)b->groups.size();
+    info->cells = b->cells;
+    info->steps_done = b->steps;
+    info->device_bytes = b->bytes;
+    info->work_items = 0;
+    for (const Group &g : b->groups)
+        info->work_items += g.nitems;
+    info->z_chunk = b->groups.empty() ? 0 : b->groups[0].L;
+    return 0;
 }
 
 } // extern "C"

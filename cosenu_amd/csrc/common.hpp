@@ -39,6 +39,32 @@ struct StageArgs {
     double dz, dv, ct, st, pmo, mu, ko;
 };
 
+// One run of a batched launch: its slices of the packed buffers and its
+// physics constants (the runs of one batch may differ in every field).
+struct BatchRun {
+    double *y, *acc, *A, *B;
+    const double *vz, *Hm;
+    int nz, nvz, VP, flags, bc, scheme;
+    double a[4]; // stage coefficients 0.5 dt, 0.5 dt, dt, dt/6
+    double dz, dv, ct, st, pmo, mu, ko;
+};
+
+// The buffer roles of RK stage `stage` (1..4), as in the single-run driver.
+__host__ __device__ inline StageArgs batch_stage_args(const BatchRun &r, int stage)
+{
+    StageArgs a;
+    a.y = r.y;
+    a.acc = r.acc;
+    a.yin = stage == 1 ? r.y : (stage == 3 ? r.B : r.A);
+    a.yout = stage == 1 ? r.A : (stage == 2 ? r.B : (stage == 3 ? r.A : r.y));
+    a.a = r.a[stage - 1];
+    a.vz = r.vz;
+    a.Hm = r.Hm;
+    a.nz = r.nz; a.nvz = r.nvz; a.VP = r.VP; a.flags = r.flags; a.bc = r.bc; a.scheme = r.scheme;
+    a.dz = r.dz; a.dv = r.dv; a.ct = r.ct; a.st = r.st; a.pmo = r.pmo; a.mu = r.mu; a.ko = r.ko;
+    return a;
+}
+
 struct RhsArgs { // plain RHS evaluation: out = RHS(in), for calRHS
     const double *in;
     double *out;
@@ -52,10 +78,16 @@ hipError_t launch_stage_strict(const StageArgs &a, int stage, hipStream_t s);
 hipError_t launch_rhs_strict(const RhsArgs &a, hipStream_t s);
 const char *stage_strict_kernel_name(int stage);
 
-// FAST mode: returns hipErrorInvalidValue for unsupported shapes.
-hipError_t launch_stage_fast(const StageArgs &a, int stage, int z_chunk, hipStream_t s);
-hipError_t launch_rhs_fast(const RhsArgs &a, int z_chunk, hipStream_t s);
-const char *stage_fast_kernel_name(int VP, int stage);
+// FAST mode: returns hipErrorInvalidValue for unsupported shapes.  variant 1
+// = one lane per velocity bin (8 fields per lane), variant 2 = two lanes per
+// bin (nu and nubar halves).
+hipError_t launch_stage_fast(const StageArgs &a, int stage, int z_chunk, int variant, hipStream_t s);
+hipError_t launch_rhs_fast(const RhsArgs &a, int z_chunk, int variant, hipStream_t s);
+const char *stage_fast_kernel_name(int VP, int stage, int variant);
+// Batched FAST stage over `nitems` work items (run index, first z cell) of
+// runs with the same VP and scheme; `runs` and `items` are device arrays.
+hipError_t launch_stage_fast_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
+                                   int scheme, int L, int stage, hipStream_t s);
 int fast_supported(int VP, int bc);
 
 // Layout conversion and reductions (aux.hip).

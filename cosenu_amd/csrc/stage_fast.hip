@@ -356,27 +356,20 @@ __device__ inline double flux_dir(const double w[kWin], bool pos)
                    pos ? w[4] : w[3], pos ? w[5] : w[2], pos ? w[6] : w[1]);
 }
 
+// One team's march over the chunk [j0, j0+L) of the run described by A.
+// `tring` is the team's LDS ring.  Teams of a block must call this with the
+// same L (they meet at one barrier per z step).
 template <int NW, int SCHEME, int STAGE>
-__global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_stage(StageArgs A, int L,
-                                                                           int nchunks)
+__device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, bool active,
+                                           double (*tring)[NW][8])
 {
-    constexpr int TPB = NW >= 4 ? 1 : 4 / NW; // teams per block
-    __shared__ double ring[TPB][8][NW][8];
-
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int team = wave / NW, wit = wave % NW;
+    const int wit = wave % NW;
     const int h = lane >> 5;
     const int v = wit * 32 + (lane & 31);
     const int VP = A.VP, nz = A.nz;
     const size_t zs = (size_t)8 * VP;
     const double sig = h ? -1.0 : 1.0;
-
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int per = nb / 8, rem = nb % 8, x = b % 8, kx = b / 8;
-    const int bb = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + kx;
-    const int chunk = bb * TPB + team;
-    const bool active = chunk < nchunks;
-    const int j0 = (active ? chunk : 0) * L;
 
     const double vi = A.vz[v];
     const bool pos = vi > 0.0;
@@ -409,7 +402,7 @@ __global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_stage(StageA
         const double t = wave_sum6(m6, lane);
         const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
         if ((lane & 7) == 0 && idx < 6)
-            ring[team][zabs & 7][wit][idx] = t;
+            tring[zabs & 7][wit][idx] = t;
     };
     if (coll) {
         moments_to_ring(4, j0);
@@ -498,7 +491,7 @@ __global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_stage(StageA
                 double M[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
                 for (int w = 0; w < NW; w++) {
-                    const double *slot = ring[team][j & 7][w];
+                    const double *slot = tring[j & 7][w];
 #pragma unroll
                     for (int m = 0; m < 6; m++)
                         M[m] += slot[m];
@@ -547,6 +540,46 @@ __global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_stage(StageA
             Fprev[f] = F[f];
         }
     }
+}
+
+// XCD-aware order: blocks are dealt round-robin to the 8 XCDs, so give each
+// XCD a contiguous run of work (neighbouring chunks share halo rows in that
+// XCD's L2).
+__device__ inline int xcd_block_order()
+{
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = nb / 8, rem = nb % 8, x = b % 8, k = b / 8;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
+}
+
+template <int NW, int SCHEME, int STAGE>
+__global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_stage(StageArgs A, int L,
+                                                                           int nchunks)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW; // teams per block
+    __shared__ double ring[TPB][8][NW][8];
+    const int team = (threadIdx.x >> 6) / NW;
+    const int chunk = xcd_block_order() * TPB + team;
+    const bool active = chunk < nchunks;
+    fast2_body<NW, SCHEME, STAGE>(A, L, (active ? chunk : 0) * L, active, ring[team]);
+}
+
+// Batched launch: work item k = (run, first z cell); every item of one launch
+// has the same chunk length L and the same padded width VP.
+template <int NW, int SCHEME, int STAGE>
+__global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_batch(const BatchRun *runs,
+                                                                           const int2 *items,
+                                                                           int nitems, int L,
+                                                                           int stage)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
+    __shared__ double ring[TPB][8][NW][8];
+    const int team = (threadIdx.x >> 6) / NW;
+    const int item = xcd_block_order() * TPB + team;
+    const bool active = item < nitems;
+    const int2 it = items[active ? item : 0];
+    const StageArgs A = batch_stage_args(runs[it.x], stage);
+    fast2_body<NW, SCHEME, STAGE>(A, L, it.y, active, ring[team]);
 }
 
 template <int NW, int SCHEME>
@@ -608,27 +641,18 @@ hipError_t launch_scheme(const StageArgs &a, int stage, int L, hipStream_t s)
 
 } // namespace
 
-int fast_variant()
-{
-    static int v = [] {
-        const char *e = getenv("COSENU_FAST_VARIANT");
-        return (e && *e == '1') ? 1 : 2;
-    }();
-    return v;
-}
-
 int fast_supported(int VP, int bc) { return (VP == 64 || VP == 128 || VP == 256) && bc == 0; }
 
-hipError_t launch_stage_fast(const StageArgs &a, int stage, int L, hipStream_t s)
+hipError_t launch_stage_fast(const StageArgs &a, int stage, int L, int variant, hipStream_t s)
 {
     if (!fast_supported(a.VP, a.bc) || L < 1 || a.nz % L != 0)
         return hipErrorInvalidValue;
-    if (fast_variant() == 1)
+    if (variant == 1)
         return a.scheme == 0 ? launch_scheme<0>(a, stage, L, s) : launch_scheme<1>(a, stage, L, s);
     return a.scheme == 0 ? launch_scheme2<0>(a, stage, L, s) : launch_scheme2<1>(a, stage, L, s);
 }
 
-hipError_t launch_rhs_fast(const RhsArgs &r, int L, hipStream_t s)
+hipError_t launch_rhs_fast(const RhsArgs &r, int L, int variant, hipStream_t s)
 {
     StageArgs a;
     a.yin = r.in; a.y = nullptr; a.acc = nullptr; a.yout = r.out;
@@ -636,14 +660,55 @@ hipError_t launch_rhs_fast(const RhsArgs &r, int L, hipStream_t s)
     a.nz = r.nz; a.nvz = r.nvz; a.VP = r.VP; a.flags = r.flags; a.bc = r.bc; a.scheme = r.scheme;
     a.a = 0.0;
     a.dz = r.dz; a.dv = r.dv; a.ct = r.ct; a.st = r.st; a.pmo = r.pmo; a.mu = r.mu; a.ko = r.ko;
-    return launch_stage_fast(a, 0, L, s);
+    return launch_stage_fast(a, 0, L, variant, s);
 }
 
-const char *stage_fast_kernel_name(int VP, int stage)
+namespace {
+template <int NW, int SCHEME>
+hipError_t launch_batch_nw(const BatchRun *runs, const int2 *items, int nitems, int L, int stage,
+                           hipStream_t s)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
+    dim3 grid((nitems + TPB - 1) / TPB), block(64 * NW * TPB);
+    if (nitems <= 0)
+        return hipSuccess;
+    switch (stage) {
+    case 1: hipLaunchKernelGGL((cosenu_fast2_batch<NW, SCHEME, 1>), grid, block, 0, s, runs, items, nitems, L, stage); break;
+    case 2:
+    case 3: hipLaunchKernelGGL((cosenu_fast2_batch<NW, SCHEME, 2>), grid, block, 0, s, runs, items, nitems, L, stage); break;
+    case 4: hipLaunchKernelGGL((cosenu_fast2_batch<NW, SCHEME, 4>), grid, block, 0, s, runs, items, nitems, L, stage); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int SCHEME>
+hipError_t launch_batch_scheme(const BatchRun *runs, const int2 *items, int nitems, int VP, int L,
+                               int stage, hipStream_t s)
+{
+    switch (VP) {
+    case 64: return launch_batch_nw<2, SCHEME>(runs, items, nitems, L, stage, s);
+    case 128: return launch_batch_nw<4, SCHEME>(runs, items, nitems, L, stage, s);
+    case 256: return launch_batch_nw<8, SCHEME>(runs, items, nitems, L, stage, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+} // namespace
+
+hipError_t launch_stage_fast_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
+                                   int scheme, int L, int stage, hipStream_t s)
+{
+    if (!fast_supported(VP, 0) || L < 1)
+        return hipErrorInvalidValue;
+    return scheme == 0 ? launch_batch_scheme<0>(runs, items, nitems, VP, L, stage, s)
+                       : launch_batch_scheme<1>(runs, items, nitems, VP, L, stage, s);
+}
+
+const char *stage_fast_kernel_name(int VP, int stage, int variant)
 {
     static char buf[96];
     const int st = stage + 1 == 3 ? 2 : stage + 1;
-    if (fast_variant() == 1)
+    if (variant == 1)
         snprintf(buf, sizeof(buf), "cosenu_fast_stage<%d, *, %d>", VP / 64, st);
     else
         snprintf(buf, sizeof(buf), "cosenu_fast2_stage<%d, *, %d>", VP / 32, st);

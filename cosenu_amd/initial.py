@@ -86,11 +86,13 @@ def seeded_ic(p: RunParams, sig_nu: float, sig_nubar: float, alpha: float, seed:
     vz = _vz(p)
     G1 = np.array([g_profile(float(v), 1.0, sig_nu) for v in vz])
     bG1 = np.array([alpha * g_profile(float(v), 1.0, sig_nubar) for v in vz])
-    phi = 2.0 * math.pi * splitmix64_uniform(seed, p.nz)
-    c, s_ = np.cos(phi), np.sin(phi)
+    # libm per cell (not numpy's SIMD kernels, whose last bit can depend on
+    # the host CPU): the state is then identical on every x86-64 host.
+    phi = (2.0 * math.pi * splitmix64_uniform(seed, p.nz)).tolist()
+    c = np.array([math.cos(x) for x in phi])
+    s_ = np.array([math.sin(x) for x in phi])
     if gaussian_seed:
-        Z = _Z(p)
-        gs = np.exp(-(Z * Z) / 50.0)
+        gs = np.array([math.exp(-(z * z) / 50.0) for z in _Z(p).tolist()])
         c = c + gs
     G = np.repeat(G1[:, None], p.nz, axis=1)
     bG = np.repeat(bG1[:, None], p.nz, axis=1)

@@ -81,6 +81,8 @@ typedef struct cosenu_info {
     long long steps_done;
     size_t device_bytes; /* device memory held by the context */
     int z_chunk;         /* FAST mode: z cells per streaming work item */
+    int fast_variant;    /* FAST mode: 1 = one lane per velocity bin, 2 = two
+                            lanes per bin (nu / nubar halves); 0 in STRICT */
 } cosenu_info;
 
 /* Diagnostics (analysis.hpp:3-185) plus conservation checks.  All sums are
@@ -129,6 +131,10 @@ int cosenu_download_planes(cosenu_ctx *ctx, int which, double *const planes[8],
 int cosenu_upload_interleaved(cosenu_ctx *ctx, int which, const double *buf);
 int cosenu_download_interleaved(cosenu_ctx *ctx, int which, double *buf);
 
+/* NuOsc::copy_state (nuosc.hpp:193-210) between the two device states, e.g.
+ * copy_state(v_stat, v_stat0) after initialize (main.cpp:139). */
+int cosenu_copy_state(cosenu_ctx *ctx, int from, int to);
+
 /* NuOsc::step_rk4 (nuosc.hpp:320-348) nsteps times.  Enqueues only. */
 int cosenu_step(cosenu_ctx *ctx, int nsteps);
 /* Block until all work of the context has finished. */
@@ -164,6 +170,48 @@ int cosenu_get_timing(cosenu_ctx *ctx, double ms[4], long long counts[4]);
 /* Name of the HIP kernel that runs RK stage s (0..3) in this context, as
  * profilers report it. */
 const char *cosenu_stage_kernel_name(cosenu_ctx *ctx, int stage);
+
+/* ------------------------------------------------------------------------
+ * Autotuner (no counterpart in the reference).  Times the FAST middle-stage
+ * kernel for each kernel variant and z-chunk length that divides nz on a
+ * scratch context of p's shape, and records the fastest; contexts created
+ * afterwards with the same (nvz rounded to 64, scheme, nz) use it.  When the
+ * environment variable COSENU_TUNE_FILE names a file, the table is read from
+ * and written to it (text: VP scheme nz variant z_chunk ms). */
+int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *ms);
+
+/* ------------------------------------------------------------------------
+ * Batched runs (no counterpart in the reference, whose sweeps are one process
+ * per job, manage.py:361-428).  Many independent FAST-mode runs, of uneven nz,
+ * nvz and physics, advance together: runs are grouped by (nvz rounded to 64,
+ * scheme), each group's states are packed into shared buffers, and one launch
+ * per RK stage and group covers all (run, z-chunk) work items; runs of
+ * cosenu_batch_step are replayed from captured HIP graphs.  Every run is
+ * bit-identical to the same run in its own context.  Per-run calls mirror the
+ * single-run ones. */
+typedef struct cosenu_batch cosenu_batch;
+typedef struct cosenu_batch_info {
+    int nruns, ngroups;
+    long long cells;      /* sum over runs of nz*nvz */
+    long long steps_done;
+    size_t device_bytes;
+    long long work_items; /* (run, z-chunk) items per stage */
+    int z_chunk;          /* chunk length of the first group */
+} cosenu_batch_info;
+
+int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out);
+int cosenu_batch_destroy(cosenu_batch *b);
+int cosenu_batch_get_info(cosenu_batch *b, cosenu_batch_info *info);
+int cosenu_batch_set_profile(cosenu_batch *b, int run, const double *G, const double *bG,
+                             ptrdiff_t row_stride, int row_offset);
+int cosenu_batch_upload_planes(cosenu_batch *b, int run, int which, const double *const planes[8],
+                               ptrdiff_t row_stride, int row_offset);
+int cosenu_batch_download_planes(cosenu_batch *b, int run, int which, double *const planes[8],
+                                 ptrdiff_t row_stride, int row_offset);
+int cosenu_batch_step(cosenu_batch *b, int nsteps);
+int cosenu_batch_sync(cosenu_batch *b);
+int cosenu_batch_diagnostics(cosenu_batch *b, int run, cosenu_diag *d);
+int cosenu_batch_survival_profile(cosenu_batch *b, int run, double *pee, double *pbee);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,103 @@
+"""Batched runs (cosenu_batch_*) and the autotuner.
+
+A batch of runs with uneven nz, nvz and physics must leave every run
+bit-identical to the same run advanced alone in its own context with the same
+FAST kernel variant (2: two lanes per velocity bin), and its diagnostics equal
+too.  Runs of one batch are grouped by (nvz rounded to 64, scheme).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cosenu_amd as C
+import oracle as O
+from cosenu_amd.initial import seeded_ic
+
+pytestmark = pytest.mark.gpu
+
+
+def _runs():
+    rng = np.random.default_rng(42)
+    specs = [(256, 64, C.FV), (448, 128, C.FV), (128, 64, C.FV), (320, 128, C.FV),
+             (192, 64, C.FD), (512, 256, C.FV), (64, 128, C.FV)]
+    runs, ics = [], []
+    for k, (nz, nvz, scheme) in enumerate(specs):
+        p = C.RunParams(nz=nz, nvz=nvz, z0=-nz / 4.0, z1=nz / 4.0, scheme=scheme,
+                        flags=C.VAC | C.COLL, omega=float(rng.uniform(0, 0.5)),
+                        theta=float(rng.uniform(1, 40)), mu=float(rng.uniform(0.5, 1.5)),
+                        mode=C.FAST)
+        ics.append(seeded_ic(p, rng.uniform(0.4, 0.8), rng.uniform(0.4, 0.8),
+                             rng.uniform(0.7, 1.1), seed=100 + k, amp=1e-3))
+        runs.append(p)
+    return runs, ics
+
+
+@pytest.fixture
+def variant2(monkeypatch):
+    monkeypatch.setenv("COSENU_FAST_VARIANT", "2")
+
+
+def test_batch_runs_bit_identical_to_solo_runs(gpu, variant2):
+    runs, ics = _runs()
+    nsteps = 20  # one 16-step graph replay + 4 eager steps
+    with C.Batch(runs) as b:
+        info = b.info()
+        assert info["nruns"] == len(runs) and info["ngroups"] == 4
+        assert info["cells"] == sum(p.nz * p.nvz for p in runs)
+        for r, (G, bG, s0) in enumerate(ics):
+            b.set_profile(r, G, bG)
+            b.upload(r, s0, C.STATE0)
+            b.upload(r, s0)
+        b.step(nsteps)
+        b.sync()
+        outs = [b.download(r) for r in range(len(runs))]
+        diags = [b.diagnostics(r) for r in range(len(runs))]
+    for r, (p, (G, bG, s0)) in enumerate(zip(runs, ics)):
+        with C.Simulation(p) as sim:
+            sim.set_profile(G, bG)
+            sim.upload(s0, C.STATE0)
+            sim.upload(s0)
+            sim.step(nsteps)
+            solo = sim.download()
+            d = sim.diagnostics()
+        assert np.array_equal(outs[r], solo), r
+        assert diags[r] == d, r
+        assert not np.array_equal(solo, s0)
+
+
+def test_batch_matches_oracle(gpu, variant2):
+    runs, ics = _runs()
+    runs, ics = runs[:3], ics[:3]
+    with C.Batch(runs) as b:
+        for r, (G, bG, s0) in enumerate(ics):
+            b.upload(r, s0)
+        b.step(10)
+        outs = [b.download(r) for r in range(len(runs))]
+    for p, (G, bG, s0), out in zip(runs, ics, outs):
+        po = O.RunParams(nz=p.nz, nvz=p.nvz, z0=p.z0, z1=p.z1, scheme=p.scheme, flags=p.flags,
+                         omega=p.omega, theta=p.theta, mu=p.mu)
+        ref = O.step(po, s0, 10)
+        assert np.max(np.abs(out - ref)) / np.max(np.abs(ref)) <= 1e-10
+
+
+def test_batch_rejects_bad_runs(gpu):
+    with pytest.raises(C.CosenuError, match="run 1"):
+        C.Batch([C.RunParams(nz=64, nvz=64), C.RunParams(nz=64, nvz=512)])
+    with pytest.raises(C.CosenuError):
+        C.Batch([])
+
+
+def test_autotune_records_and_applies(gpu, tmp_path, monkeypatch):
+    table = tmp_path / "tune.txt"
+    monkeypatch.setenv("COSENU_TUNE_FILE", str(table))
+    monkeypatch.delenv("COSENU_FAST_VARIANT", raising=False)
+    monkeypatch.delenv("COSENU_ZCHUNK", raising=False)
+    p = C.RunParams(nz=4096, nvz=128, mode=C.FAST)
+    v, L, ms = C.autotune(p)
+    assert v in (1, 2) and L > 0 and 4096 % L == 0 and ms > 0
+    rows = [l.split() for l in table.read_text().splitlines() if not l.startswith("#")]
+    assert ["128", "0", "4096", str(v), str(L)] == rows[-1][:5]
+    with C.Simulation(p) as sim:
+        info = sim.info()
+    assert info["fast_variant"] == v and info["z_chunk"] == L

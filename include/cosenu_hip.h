@@ -213,6 +213,31 @@ int cosenu_batch_sync(cosenu_batch *b);
 int cosenu_batch_diagnostics(cosenu_batch *b, int run, cosenu_diag *d);
 int cosenu_batch_survival_profile(cosenu_batch *b, int run, double *pee, double *pbee);
 
+/* ------------------------------------------------------------------------
+ * Learned surrogate (no counterpart in the reference): a fully connected
+ * network dims[0] -> ... -> dims[nlayers] with GELU between layers, BF16
+ * weights and activations, FP32 accumulation on the matrix cores, FP32
+ * master weights and AdamW state on the device.  Parameters are exchanged as
+ * one flat FP32 array, layer by layer: W_l (dims[l] x dims[l+1], row-major,
+ * y = x W + b) then b_l.  Inputs X are M x dims[0], outputs/targets
+ * M x dims[nlayers], FP32 row-major host arrays.  Errors: negative status and
+ * cosenu_mlp_last_error(). */
+typedef struct cosenu_mlp cosenu_mlp;
+const char *cosenu_mlp_last_error(void);
+int cosenu_mlp_create(const int *dims, int nlayers, int max_batch, int device, cosenu_mlp **out);
+int cosenu_mlp_destroy(cosenu_mlp *m);
+long long cosenu_mlp_num_params(cosenu_mlp *m);
+int cosenu_mlp_set_params(cosenu_mlp *m, const float *flat);
+int cosenu_mlp_get_params(cosenu_mlp *m, float *flat);
+int cosenu_mlp_forward(cosenu_mlp *m, const float *X, int M, float *Y);
+/* mean squared error over all M x dims[nlayers] outputs and its gradient */
+int cosenu_mlp_gradients(cosenu_mlp *m, const float *X, const float *T, int M, float *grad,
+                         float *loss);
+/* one AdamW step (decoupled weight decay, none on biases) on that loss */
+int cosenu_mlp_train_step(cosenu_mlp *m, const float *X, const float *T, int M, float lr,
+                          float beta1, float beta2, float eps, float weight_decay, float *loss);
+int cosenu_mlp_sync(cosenu_mlp *m);
+
 #ifdef __cplusplus
 }
 #endif

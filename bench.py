@@ -93,13 +93,12 @@ def main():
 
     wl = workload(args.workload, mode=C.FAST, scheme=C.FV if args.scheme == "fv" else C.FD)
     p = wl.params
-    G, bG, s0 = wl.ic()
+    a, b, G1, bG1 = wl.factors()  # the separable initial state, built on the device
     cells = p.nz * p.nvz
     sim = C.Simulation(p)
-    sim.set_profile(G, bG)
-    sim.upload(s0, C.STATE0)
-    sim.upload(s0)
-    del s0, G, bG
+    sim.set_profile_v(G1, bG1)
+    sim.init_separable(a, b, C.STATE0)
+    sim.init_separable(a, b, C.STATE)
 
     sim.step(args.warmup)
     sim.sync()

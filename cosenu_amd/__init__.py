@@ -103,6 +103,8 @@ def lib() -> ctypes.CDLL:
     L.cosenu_stage_kernel_name.argtypes = [vp, ctypes.c_int]
     L.cosenu_stage_kernel_name.restype = ctypes.c_char_p
     L.cosenu_copy_state.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.cosenu_init_separable.argtypes = [vp, ctypes.c_int, _dp, _dp]
+    L.cosenu_set_profile_v.argtypes = [vp, _dp, _dp]
     ip = ctypes.POINTER(ctypes.c_int)
     L.cosenu_autotune.argtypes = [ctypes.POINTER(_Params), ip, ip, ctypes.POINTER(ctypes.c_double)]
     L.cosenu_batch_create.argtypes = [ctypes.POINTER(_Params), ctypes.c_int, ctypes.POINTER(vp)]
@@ -269,6 +271,19 @@ class Simulation:
         i = _Info()
         _check(lib().cosenu_get_info(self._h, ctypes.byref(i)), "cosenu_get_info")
         return {k: getattr(i, k) for k, _ in _Info._fields_}
+
+    def init_separable(self, a: np.ndarray, b: np.ndarray, which: int = STATE):
+        """state[f, v, z] = a[f, v] * b[f, z] built on the device."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        assert a.shape == (8, self.p.nvz) and b.shape == (8, self.p.nz)
+        _check(lib().cosenu_init_separable(self._h, which, _ptr(a), _ptr(b)), "init_separable")
+
+    def set_profile_v(self, G: np.ndarray, bG: np.ndarray):
+        G = np.ascontiguousarray(G, dtype=np.float64)
+        bG = np.ascontiguousarray(bG, dtype=np.float64)
+        assert G.shape == (self.p.nvz,) and bG.shape == (self.p.nvz,)
+        _check(lib().cosenu_set_profile_v(self._h, _ptr(G), _ptr(bG)), "set_profile_v")
 
     def copy_state(self, src: int, dst: int):
         _check(lib().cosenu_copy_state(self._h, src, dst), "cosenu_copy_state")

@@ -645,6 +645,44 @@ int cosenu_download_interleaved(cosenu_ctx *c, int which, double *buf)
     return view_download_interleaved(c->view, which, buf, c->stream);
 }
 
+int cosenu_set_profile_v(cosenu_ctx *c, const double *G, const double *bG)
+{
+    if (!c || !G || !bG)
+        return fail(COSENU_EINVAL, "cosenu_set_profile_v: null argument");
+    HIP_TRY(hipSetDevice(c->p.device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // per-bin values staged at the start of the (idle) stage buffer A
+    HIP_TRY(hipMemcpyAsync(c->A, G, c->nvz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->A + c->VP, bG, c->nvz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_profile_v(c->prof, c->A, c->A + c->VP, c->nz, c->nvz, c->VP, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->have_profile = true;
+    return 0;
+}
+
+int cosenu_init_separable(cosenu_ctx *c, int which, const double *a, const double *b)
+{
+    if (!c || !a || !b || (which != COSENU_STATE && which != COSENU_STATE0))
+        return fail(COSENU_EINVAL, "cosenu_init_separable: bad argument");
+    HIP_TRY(hipSetDevice(c->p.device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // factors staged in the (idle) stage buffer A: 8*VP per-bin, then 8*nz per-z
+    std::vector<double> av((size_t)8 * c->VP, 0.0);
+    for (int f = 0; f < 8; f++)
+        for (int v = 0; v < c->nvz; v++)
+            av[(size_t)f * c->VP + v] = a[(size_t)f * c->nvz + v];
+    HIP_TRY(hipMemcpyAsync(c->A, av.data(), av.size() * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->A + av.size(), b, (size_t)8 * c->nz * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_separable(which_ptr(c->view, which), c->A, c->A + av.size(), c->nz, c->nvz, c->VP,
+                             c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (which == COSENU_STATE0)
+        c->have_state0 = true;
+    return 0;
+}
+
 int cosenu_copy_state(cosenu_ctx *c, int from, int to)
 {
     if (!c || (from != COSENU_STATE && from != COSENU_STATE0) ||

@@ -228,7 +228,47 @@ __global__ void k_surv_final(const double *partials, int npart, int VP, double *
 
 int rows_per_block(int nz, int npart) { return (nz + npart - 1) / npart; }
 
+// state(z, f, v) = a[f*VP + v] * b[f*nz + z] (one rounding, like the host's
+// product); padding bins get zero.
+__global__ __launch_bounds__(256) void k_separable(double *dst, const double *a, const double *b,
+                                                   int nz, int nvz, int VP)
+{
+    const long n = (long)nz * 8 * VP;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int v = (int)(i % VP);
+        const long zf = i / VP;
+        const int f = (int)(zf % 8), z = (int)(zf / 8);
+        dst[i] = v < nvz ? a[f * VP + v] * b[(long)f * nz + z] : 0.0;
+    }
+}
+
+// profile rows [z][2][VP] from per-bin G, bG
+__global__ __launch_bounds__(256) void k_profile_v(double *dst, const double *G, const double *bG,
+                                                   int nz, int nvz, int VP)
+{
+    const long n = (long)nz * 2 * VP;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int v = (int)(i % VP);
+        const int k = (int)((i / VP) % 2);
+        dst[i] = v < nvz ? (k ? bG[v] : G[v]) : 0.0;
+    }
+}
+
 } // namespace
+
+hipError_t launch_separable(double *dst, const double *a, const double *b, int nz, int nvz, int VP,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(k_separable, dim3(4096), dim3(256), 0, s, dst, a, b, nz, nvz, VP);
+    return hipGetLastError();
+}
+
+hipError_t launch_profile_v(double *dst, const double *G, const double *bG, int nz, int nvz, int VP,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(k_profile_v, dim3(2048), dim3(256), 0, s, dst, G, bG, nz, nvz, VP);
+    return hipGetLastError();
+}
 
 hipError_t launch_import_plane(double *dst, const double *src, int f, int nz, int nvz, int VP,
                                ptrdiff_t stride, int off, hipStream_t s)

@@ -114,5 +114,10 @@ hipError_t launch_survival_profile(const double *y, const double *y0, int nz, in
                                    double *partials, int npart, double *out, hipStream_t s);
 hipError_t launch_pee_plane(double *dst, const double *y, const double *y0, int nz, int nvz,
                             int VP, hipStream_t s);
+// state(z, f, v) = a[f*VP + v] * b[f*nz + z]; a, b on the device
+hipError_t launch_separable(double *dst, const double *a, const double *b, int nz, int nvz, int VP,
+                            hipStream_t s);
+hipError_t launch_profile_v(double *dst, const double *G, const double *bG, int nz, int nvz, int VP,
+                            hipStream_t s);
 
 } // namespace cosenu

@@ -74,7 +74,13 @@ __device__ inline double weno_up(double a0, double a1, double a2, double a3, dou
     double e0 = 4.0 * (q1 * p23), e1 = 18.0 * (q0 * p23), e2 = 12.0 * (p01 * q3), e3 = p01 * q2;
     double num = e0 * c0 + e1 * c1 + e2 * c2 + e3 * c3;
     double den = (e0 + e1) + (e2 + e3);
-    return num / den;
+    // num/den: hardware reciprocal, one Newton step, then a residual
+    // correction of the quotient (within an ulp of the IEEE quotient; den is
+    // a positive normal number, at least (1e-6)^6).
+    double r = __builtin_amdgcn_rcp(den);
+    r = fma(r, fma(-den, r, 1.0), r);
+    const double qt = num * r;
+    return fma(r, fma(-den, qt, num), qt);
 }
 
 template <int W, int SCHEME, int STAGE>

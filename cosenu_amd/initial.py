@@ -80,9 +80,10 @@ def splitmix64_uniform(seed: int, n: int) -> np.ndarray:
     return (out >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
 
-def seeded_ic(p: RunParams, sig_nu: float, sig_nubar: float, alpha: float, seed: int,
-              amp: float = 1e-6, gaussian_seed: bool = False):
-    """(G, bG, state) of a flavour-unstable ELN state with random phases per z."""
+def seeded_factors(p: RunParams, sig_nu: float, sig_nubar: float, alpha: float, seed: int,
+                   amp: float = 1e-6, gaussian_seed: bool = False):
+    """Separable form of the seeded state: field f at (v, z) is a[f, v] * b[f, z].
+    Returns (a [8, nvz], b [8, nz], G(v), bG(v))."""
     vz = _vz(p)
     G1 = np.array([g_profile(float(v), 1.0, sig_nu) for v in vz])
     bG1 = np.array([alpha * g_profile(float(v), 1.0, sig_nubar) for v in vz])
@@ -94,15 +95,21 @@ def seeded_ic(p: RunParams, sig_nu: float, sig_nubar: float, alpha: float, seed:
     if gaussian_seed:
         gs = np.array([math.exp(-(z * z) / 50.0) for z in _Z(p).tolist()])
         c = c + gs
+    one = np.ones(p.nz)
+    zero = np.zeros(p.nz)
+    a = np.stack([G1, np.zeros_like(G1), amp * G1, amp * G1,
+                  bG1, np.zeros_like(G1), amp * bG1, amp * bG1])
+    b = np.stack([one, zero, c, s_, one, zero, c, s_])
+    return a, b, G1, bG1
+
+
+def seeded_ic(p: RunParams, sig_nu: float, sig_nubar: float, alpha: float, seed: int,
+              amp: float = 1e-6, gaussian_seed: bool = False):
+    """(G, bG, state) of a flavour-unstable ELN state with random phases per z."""
+    a, b, G1, bG1 = seeded_factors(p, sig_nu, sig_nubar, alpha, seed, amp, gaussian_seed)
+    st = a[:, :, None] * b[:, None, :]
     G = np.repeat(G1[:, None], p.nz, axis=1)
     bG = np.repeat(bG1[:, None], p.nz, axis=1)
-    st = np.zeros((8, p.nvz, p.nz))
-    st[0] = G
-    st[2] = amp * G * c[None, :]
-    st[3] = amp * G * s_[None, :]
-    st[4] = bG
-    st[6] = amp * bG * c[None, :]
-    st[7] = amp * bG * s_[None, :]
     return G, bG, st
 
 
@@ -120,6 +127,10 @@ class Workload:
     def ic(self):
         return seeded_ic(self.params, self.sig_nu, self.sig_nubar, self.alpha, self.seed,
                          gaussian_seed=self.gaussian_seed)
+
+    def factors(self):
+        return seeded_factors(self.params, self.sig_nu, self.sig_nubar, self.alpha, self.seed,
+                              gaussian_seed=self.gaussian_seed)
 
 
 def workload(name: str, mode: int = FAST, scheme: int = FV) -> Workload:

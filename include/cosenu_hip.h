@@ -131,6 +131,15 @@ int cosenu_download_planes(cosenu_ctx *ctx, int which, double *const planes[8],
 int cosenu_upload_interleaved(cosenu_ctx *ctx, int which, const double *buf);
 int cosenu_download_interleaved(cosenu_ctx *ctx, int which, double *buf);
 
+/* Initial states built on the device (the reference builds them on the host,
+ * initialize.hpp:214-234).  Every initial state of the reference and of the
+ * benchmark workloads is separable: field f at (z, v) is a[f*nvz + v] *
+ * b[f*nz + z], one rounded product, exactly what the host computes (e.g.
+ * ee = (0.5 G(v)) (1 + sqrt(1 - eps(z)^2))).  a is 8 x nvz, b is 8 x nz.
+ * cosenu_set_profile_v takes z-independent profiles G(v), bG(v) (nvz each). */
+int cosenu_init_separable(cosenu_ctx *ctx, int which, const double *a, const double *b);
+int cosenu_set_profile_v(cosenu_ctx *ctx, const double *G, const double *bG);
+
 /* NuOsc::copy_state (nuosc.hpp:193-210) between the two device states, e.g.
  * copy_state(v_stat, v_stat0) after initialize (main.cpp:139). */
 int cosenu_copy_state(cosenu_ctx *ctx, int from, int to);

@@ -244,3 +244,22 @@ def test_errors_are_reported_not_fatal(gpu):
     with C.Simulation(C.RunParams(nz=64, nvz=16, mode=C.STRICT)) as sim:
         with pytest.raises(C.CosenuError, match="profile"):
             sim.diagnostics()
+
+
+def test_separable_initial_state_equals_host_state(gpu):
+    """cosenu_init_separable / cosenu_set_profile_v build on the device what
+    the host computes (one rounded product per value)."""
+    from cosenu_amd.initial import seeded_factors
+    po, pc = _pair(O.FV, O.PERIODIC, O.VAC | O.COLL, 96, 40, C.FAST)
+    a, b, G1, bG1 = seeded_factors(pc, 0.6, 0.5, 0.9, seed=21, amp=1e-3, gaussian_seed=True)
+    G, bG, s0 = seeded_ic(pc, 0.6, 0.5, 0.9, seed=21, amp=1e-3, gaussian_seed=True)
+    with C.Simulation(pc) as sim:
+        sim.init_separable(a, b, C.STATE)
+        sim.init_separable(a, b, C.STATE0)
+        assert np.array_equal(sim.download(), s0)
+        assert np.array_equal(sim.download(C.STATE0), s0)
+        sim.set_profile_v(G1, bG1)
+        d1 = sim.diagnostics()
+        sim.set_profile(G, bG)
+        d2 = sim.diagnostics()
+    assert d1 == d2

@@ -212,12 +212,8 @@ int validate(const cosenu_params *p)
     if (!(p->z1 > p->z0) || !(p->v1 > p->v0) || !(p->CFL > 0))
         return fail(COSENU_EINVAL, "need z1 > z0, v1 > v0, CFL > 0");
     const int VP = ((p->nvz + 63) / 64) * 64;
-    if (p->mode == COSENU_FAST) {
-        if (!fast_supported(VP, p->bc))
-            return fail(COSENU_ENOTSUP, "FAST mode supports periodic runs with nvz up to 256");
-        if ((p->flags & COSENU_KO5) != 0)
-            return fail(COSENU_ENOTSUP, "FAST mode implements the 3rd-order KO dissipation only");
-    }
+    if (p->mode == COSENU_FAST && !fast_supported(VP, p->bc))
+        return fail(COSENU_ENOTSUP, "FAST mode supports nvz up to 256");
     return 0;
 }
 
@@ -501,6 +497,8 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             c->variant = ev;
         if (ez > 0 && ez <= c->nz && c->nz % ez == 0)
             c->z_chunk = ez;
+        if (p->bc == COSENU_OPEN)
+            c->variant = 2; // the open boundary exists in the split-lane kernel only
     }
 
     int rc = 0;
@@ -884,7 +882,7 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = fail(COSENU_EHIP, "cosenu_autotune: hipEventCreate failed");
     const int cands[] = {8, 16, 32, 64, 128, 256};
-    for (int var = 1; var <= 2 && !rc; var++)
+    for (int var = q.bc == COSENU_OPEN ? 2 : 1; var <= 2 && !rc; var++)
         for (int L : cands) {
             if (rc || L > c->nz || c->nz % L)
                 continue;
@@ -1033,6 +1031,8 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
             return fail(rc, "run " + std::to_string(r) + ": " + g_err);
         if (runs[r].device != runs[0].device)
             return fail(COSENU_EINVAL, "all runs of a batch must use one device");
+        if (runs[r].bc != COSENU_PERIODIC)
+            return fail(COSENU_ENOTSUP, "run " + std::to_string(r) + ": batches take periodic runs");
     }
     HIP_TRY(hipSetDevice(runs[0].device));
     cosenu_batch *b = new cosenu_batch();

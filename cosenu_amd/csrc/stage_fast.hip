@@ -125,10 +125,12 @@ __global__ __launch_bounds__(256) void cosenu_fast_stage(StageArgs A, int L, int
             win[f][kk] = r[(size_t)f * VP];
     }
 
+    // padding bins (v >= nvz) may hold stale staging data: keep them out
+    const bool live = v < A.nvz;
     auto moments_to_ring = [&](int kk, int zabs) {
-        double S = win[EXI][kk] + win[BEXI][kk];
-        double R = win[EXR][kk] - win[BEXR][kk];
-        double D = (win[EE][kk] - win[XX][kk]) - (win[BEE][kk] - win[BXX][kk]);
+        double S = live ? win[EXI][kk] + win[BEXI][kk] : 0.0;
+        double R = live ? win[EXR][kk] - win[BEXR][kk] : 0.0;
+        double D = live ? (win[EE][kk] - win[XX][kk]) - (win[BEE][kk] - win[BXX][kk]) : 0.0;
         double m0 = wave_sum(S), m1 = wave_sum(vi * S);
         double m2 = wave_sum(R), m3 = wave_sum(vi * R);
         double m4 = wave_sum(D), m5 = wave_sum(vi * D);
@@ -364,8 +366,11 @@ __device__ inline double flux_dir(const double w[kWin], bool pos)
 
 // One team's march over the chunk [j0, j0+L) of the run described by A.
 // `tring` is the team's LDS ring.  Teams of a block must call this with the
-// same L (they meet at one barrier per z step).
-template <int NW, int SCHEME, int STAGE>
+// same L (they meet at one barrier per z step).  OPENB selects the open
+// boundary of nuosc.hpp:247-277: rows below 0 copy row 0, rows above nz-1 are
+// extrapolated linearly one after another, and the flux ghosts follow the
+// same rule (rflux[-1] = rflux[0]; lflux[nz] = 2 lflux[nz-1] - lflux[nz-2]).
+template <int NW, int SCHEME, int STAGE, bool OPENB>
 __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, bool active,
                                            double (*tring)[NW][8])
 {
@@ -386,24 +391,55 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, bo
     const double *in = A.yin + (size_t)(4 * h) * VP + v; // this lane's first field
 
     auto zrow = [&](int m) -> int {
+        if (OPENB)
+            return m;
         int w = m % nz;
         return w < 0 ? w + nz : w;
+    };
+    // the 4 fields of this lane at z row m (m is wave-uniform)
+    auto load4 = [&](int m, double o[4]) {
+        if (!OPENB || (m >= 0 && m < nz)) {
+            const double *r = in + (size_t)zrow(m) * zs;
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                o[f] = r[(size_t)f * VP];
+        } else if (m < 0) {
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                o[f] = in[(size_t)f * VP];
+        } else {
+            const double *r1 = in + (size_t)(nz - 2) * zs, *r2 = in + (size_t)(nz - 1) * zs;
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                double a = r1[(size_t)f * VP], b = r2[(size_t)f * VP];
+                for (int k = nz; k <= m; k++) {
+                    const double c = b * 2 - a;
+                    a = b;
+                    b = c;
+                }
+                o[f] = b;
+            }
+        }
     };
 
     double win[4][kWin];
 #pragma unroll
     for (int kk = 0; kk < kWin; kk++) {
-        const double *r = in + (size_t)zrow(j0 - 4 + kk) * zs;
+        double t4[4];
+        load4(j0 - 4 + kk, t4);
 #pragma unroll
         for (int f = 0; f < 4; f++)
-            win[f][kk] = r[(size_t)f * VP];
+            win[f][kk] = t4[f];
     }
 
     const bool coll = (A.flags & kFlagColl) != 0;
+    // Padding bins (v >= nvz) hold whatever a staging copy left in the stage
+    // buffers and are never written: keep them out of the moments.
+    const bool live = v < A.nvz;
     auto moments_to_ring = [&](int kk, int zabs) {
-        const double S = win[3][kk];
-        const double R = sig * win[2][kk];
-        const double D = sig * (win[0][kk] - win[1][kk]);
+        const double S = live ? win[3][kk] : 0.0;
+        const double R = live ? sig * win[2][kk] : 0.0;
+        const double D = live ? sig * (win[0][kk] - win[1][kk]) : 0.0;
         const double m6[6] = {S, vi * S, R, vi * R, D, vi * D};
         const double t = wave_sum6(m6, lane);
         const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
@@ -418,22 +454,19 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, bo
 
     const double fac = -vi / A.dz;
     const double fdfac = -vi / (12.0 * A.dz);
-    const double ko_eps = -A.ko / A.dz / 16.0;
+    const bool ko5 = (A.flags & kFlagKO5) != 0;
+    const double ko_eps = ko5 ? -A.ko / A.dz / 64.0 : -A.ko / A.dz / 16.0;
     const double mudv = A.mu * A.dv;
     const double st = A.st * A.pmo, ct = A.ct * A.pmo;
     const bool vac = (A.flags & kFlagVac) != 0, mat = (A.flags & kFlagMat) != 0;
     const bool advect = !(A.flags & kFlagAdvecOff);
 
     double Fprev[4] = {0.0, 0.0, 0.0, 0.0};
+    double Fprev2[4] = {0.0, 0.0, 0.0, 0.0}; // open boundary only
 
     for (int j = j0 - 1; j < j0 + L; j++) {
         double nxt[4];
-        {
-            const double *r = in + (size_t)zrow(j + 5) * zs;
-#pragma unroll
-            for (int f = 0; f < 4; f++)
-                nxt[f] = r[(size_t)f * VP];
-        }
+        load4(j + 5, nxt);
         const bool cell = j >= j0;
         const size_t q = (size_t)zrow(j) * zs + (size_t)(4 * h) * VP + v;
         double yv[4], av[4];
@@ -472,8 +505,14 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, bo
             if (advect) {
                 if (SCHEME == 0) {
 #pragma unroll
-                    for (int f = 0; f < 4; f++)
-                        K[f] = fac * (F[f] - Fprev[f]);
+                    for (int f = 0; f < 4; f++) {
+                        double Fr = F[f], Fl = Fprev[f];
+                        if (OPENB && j == 0 && pos)
+                            Fl = Fr; // rflux[-1] = rflux[0]
+                        if (OPENB && j == nz - 1 && !pos)
+                            Fr = Fprev[f] * 2 - Fprev2[f]; // lflux[nz] extrapolated
+                        K[f] = fac * (Fr - Fl);
+                    }
                 } else {
 #pragma unroll
                     for (int f = 0; f < 4; f++) {
@@ -514,8 +553,12 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, bo
             if (SCHEME == 1) {
 #pragma unroll
                 for (int f = 0; f < 4; f++) {
-                    const double *w = win[f];
-                    K[f] += ko_eps * ((w[1] + w[5]) - 4.0 * (w[2] + w[4]) + 6.0 * w[3]);
+                    const double *w = win[f]; // w[k] = u(j-3+k)
+                    if (ko5)
+                        K[f] += ko_eps * ((w[0] + w[6]) - 6.0 * (w[1] + w[5]) + 15.0 * (w[2] + w[4]) -
+                                          20.0 * w[3]);
+                    else
+                        K[f] += ko_eps * ((w[1] + w[5]) - 4.0 * (w[2] + w[4]) + 6.0 * w[3]);
                 }
             }
             if (store) {
@@ -543,6 +586,8 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, bo
             for (int kk = 0; kk < kWin - 1; kk++)
                 win[f][kk] = win[f][kk + 1];
             win[f][kWin - 1] = nxt[f];
+            if (OPENB)
+                Fprev2[f] = Fprev[f];
             Fprev[f] = F[f];
         }
     }
@@ -558,7 +603,7 @@ __device__ inline int xcd_block_order()
     return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
 }
 
-template <int NW, int SCHEME, int STAGE>
+template <int NW, int SCHEME, int STAGE, bool OPENB>
 __global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_stage(StageArgs A, int L,
                                                                            int nchunks)
 {
@@ -567,7 +612,7 @@ __global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_stage(StageA
     const int team = (threadIdx.x >> 6) / NW;
     const int chunk = xcd_block_order() * TPB + team;
     const bool active = chunk < nchunks;
-    fast2_body<NW, SCHEME, STAGE>(A, L, (active ? chunk : 0) * L, active, ring[team]);
+    fast2_body<NW, SCHEME, STAGE, OPENB>(A, L, (active ? chunk : 0) * L, active, ring[team]);
 }
 
 // Batched launch: work item k = (run, first z cell); every item of one launch
@@ -585,33 +630,34 @@ __global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_batch(const 
     const bool active = item < nitems;
     const int2 it = items[active ? item : 0];
     const StageArgs A = batch_stage_args(runs[it.x], stage);
-    fast2_body<NW, SCHEME, STAGE>(A, L, it.y, active, ring[team]);
+    fast2_body<NW, SCHEME, STAGE, false>(A, L, it.y, active, ring[team]);
 }
 
-template <int NW, int SCHEME>
+template <int NW, int SCHEME, bool OPENB>
 hipError_t launch_nw(const StageArgs &a, int stage, int L, hipStream_t s)
 {
     constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
     const int nchunks = a.nz / L;
     dim3 grid((nchunks + TPB - 1) / TPB), block(64 * NW * TPB);
     switch (stage) {
-    case 0: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 0>), grid, block, 0, s, a, L, nchunks); break;
-    case 1: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 1>), grid, block, 0, s, a, L, nchunks); break;
+    case 0: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 0, OPENB>), grid, block, 0, s, a, L, nchunks); break;
+    case 1: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 1, OPENB>), grid, block, 0, s, a, L, nchunks); break;
     case 2:
-    case 3: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 2>), grid, block, 0, s, a, L, nchunks); break;
-    case 4: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 4>), grid, block, 0, s, a, L, nchunks); break;
+    case 3: // stages 2 and 3 run the same code with their own coefficient
+        hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 2, OPENB>), grid, block, 0, s, a, L, nchunks); break;
+    case 4: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 4, OPENB>), grid, block, 0, s, a, L, nchunks); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <int SCHEME>
+template <int SCHEME, bool OPENB>
 hipError_t launch_scheme2(const StageArgs &a, int stage, int L, hipStream_t s)
 {
     switch (a.VP) {
-    case 64: return launch_nw<2, SCHEME>(a, stage, L, s);
-    case 128: return launch_nw<4, SCHEME>(a, stage, L, s);
-    case 256: return launch_nw<8, SCHEME>(a, stage, L, s);
+    case 64: return launch_nw<2, SCHEME, OPENB>(a, stage, L, s);
+    case 128: return launch_nw<4, SCHEME, OPENB>(a, stage, L, s);
+    case 256: return launch_nw<8, SCHEME, OPENB>(a, stage, L, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -647,15 +693,19 @@ hipError_t launch_scheme(const StageArgs &a, int stage, int L, hipStream_t s)
 
 } // namespace
 
-int fast_supported(int VP, int bc) { return (VP == 64 || VP == 128 || VP == 256) && bc == 0; }
+int fast_supported(int VP, int bc) { return (VP == 64 || VP == 128 || VP == 256) && (bc == 0 || bc == 1); }
 
 hipError_t launch_stage_fast(const StageArgs &a, int stage, int L, int variant, hipStream_t s)
 {
     if (!fast_supported(a.VP, a.bc) || L < 1 || a.nz % L != 0)
         return hipErrorInvalidValue;
+    if (a.bc == 1) // open boundary: split-lane kernel only
+        return a.scheme == 0 ? launch_scheme2<0, true>(a, stage, L, s)
+                             : launch_scheme2<1, true>(a, stage, L, s);
     if (variant == 1)
         return a.scheme == 0 ? launch_scheme<0>(a, stage, L, s) : launch_scheme<1>(a, stage, L, s);
-    return a.scheme == 0 ? launch_scheme2<0>(a, stage, L, s) : launch_scheme2<1>(a, stage, L, s);
+    return a.scheme == 0 ? launch_scheme2<0, false>(a, stage, L, s)
+                         : launch_scheme2<1, false>(a, stage, L, s);
 }
 
 hipError_t launch_rhs_fast(const RhsArgs &r, int L, int variant, hipStream_t s)

@@ -108,9 +108,12 @@ def test_fast_rhs_normwise(gpu, scheme, flags, nvz):
             assert _rel(out, ref) <= tol
 
 
-@pytest.mark.parametrize("nvz", [64, 128, 256])
+@pytest.mark.parametrize("nvz", [16, 40, 64, 100, 128, 256])
 @pytest.mark.parametrize("scheme", [O.FV, O.FD])
 def test_fast_rk4_normwise(gpu, scheme, nvz):
+    """nvz not a multiple of 64 leaves padding bins in the device layout;
+    uploads stage host data through the stage buffers, so those bins hold
+    stale values that must never reach the moments (regression test)."""
     po, pc = _pair(scheme, O.PERIODIC, O.VAC | O.COLL, 128, nvz, C.FAST)
     _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=15, amp=1e-3)
     ref = O.step(po, s, 30)
@@ -263,3 +266,24 @@ def test_separable_initial_state_equals_host_state(gpu):
         sim.set_profile(G, bG)
         d2 = sim.diagnostics()
     assert d1 == d2
+
+
+@pytest.mark.parametrize("scheme,bc,flags", [(O.FV, O.OPEN, O.VAC | O.COLL),
+                                             (O.FD, O.OPEN, O.VAC | O.COLL),
+                                             (O.FD, O.PERIODIC, O.VAC | O.COLL | O.KO5),
+                                             (O.FV, O.OPEN, O.VAC | O.MAT | O.COLL)])
+@pytest.mark.parametrize("nvz", [24, 128])
+def test_fast_open_boundary_and_ko5(gpu, scheme, bc, flags, nvz):
+    """FAST mode with the open boundary of nuosc.hpp:247-277 (ghost rows and
+    flux ghosts) and the 5th-order Kreiss-Oliger term of rhs_fd.hpp:129-133."""
+    po, pc = _pair(scheme, bc, flags, 96, nvz, C.FAST)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=22, amp=1e-3)
+    # a z-dependent background so the boundary rules matter
+    s = s * (1.0 + 0.05 * np.sin(np.linspace(0, 3.0, po.nz)))[None, None, :]
+    with C.Simulation(pc) as sim:
+        sim.upload(s)
+        assert _rel(sim.rhs(), O.rhs(po, s)) <= 1e-12
+        sim.upload(s)
+        sim.step(30)
+        out = sim.download()
+    assert _rel(out, O.step(po, s, 30)) <= 1e-10

@@ -26,8 +26,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FP64 z·v cell-updates/s + HBM roofline %; surrogate vs solver P_ee sweep speedup"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL_STEP = 1088           # y, stage input, accumulator: 3 x 8 FP64 per stage
-BYTES_PER_CELL_MID_STAGE = 320       # stages 2 and 3: read Y_s, y, acc; write acc, Y_s+1
+# Bytes per cell of one RK4 step under the classic accumulator form (read
+# the stage input, y and the accumulator, write the accumulator and the next
+# stage input, 8 FP64 fields each): the figure the roofline is defined with.
+BYTES_PER_CELL_STEP_DEF = 1088
+# Algorithmic bytes per cell of each stage kernel, by FAST kernel variant:
+#   1 (one lane per bin, classic form): 64+128, 192+128, 192+128, 192+64
+#   2 (two lanes per bin, stage-value form: no accumulator; stage 4 reads
+#      y, Y2, Y3 pointwise): 64+64, 128+64, 128+64, 256+64
+STAGE_BYTES = {1: (192, 320, 320, 256), 2: (128, 192, 192, 320)}
 
 
 def cpu_baseline(seconds_target: float = 12.0):
@@ -113,17 +120,26 @@ def main():
     value = cells * args.steps / el
     ms_step = 1e3 * el / args.steps
 
-    # Dominant kernel: the middle-stage kernel (RK stages 2 and 3).
-    mid_ms = (ms[1] + ms[2]) / max(1, cnt[1] + cnt[2])
-    mid_bytes = BYTES_PER_CELL_MID_STAGE * cells
-    achieved = mid_bytes / (mid_ms * 1e-3) / 1e9
-    step_kernel_ms = sum(ms) / max(1, cnt[0])
-    step_gbs = BYTES_PER_CELL_STEP * cells / (step_kernel_ms * 1e-3) / 1e9
+    # Per-kernel times from HIP events on the library's stream.  Stages 2 and
+    # 3 share one kernel; the dominant kernel is the one with the largest
+    # total time per step.
+    variant = sim.info()["fast_variant"]
+    sb = STAGE_BYTES[variant]
+    avg = [m / max(1, c) for m, c in zip(ms, cnt)]
+    kernels = {"stage1": (avg[0], sb[0], sim.stage_kernel_name(0), 1),
+               "stage2+3": ((ms[1] + ms[2]) / max(1, cnt[1] + cnt[2]), sb[1],
+                            sim.stage_kernel_name(1), 2),
+               "stage4": (avg[3], sb[3], sim.stage_kernel_name(3), 1)}
+    dom = max(kernels, key=lambda k: kernels[k][0] * kernels[k][3])
+    dom_ms, dom_bytes, dom_name, _ = kernels[dom]
+    achieved = dom_bytes * cells / (dom_ms * 1e-3) / 1e9
+    step_kernel_ms = sum(avg)
+    step_gbs = sum(sb) * cells / (step_kernel_ms * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.scheme}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("mid_stage_hbm_bytes_per_launch")
+            traffic = json.load(f).get(dom)
 
     out = {
         "metric": METRIC,
@@ -144,11 +160,14 @@ def main():
                    "z_chunk": sim.info()["z_chunk"], "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": sim.stage_kernel_name(1),
-                     "kernel_avg_ms": mid_ms,
-                     "step_all_stages_GBs": step_gbs,
-                     "step_all_stages_frac": step_gbs / HBM_PEAK_GBS,
-                     "stage_avg_ms": [m / max(1, c) for m, c in zip(ms, cnt)]},
+                     "kernel": dom_name, "kernel_stage": dom,
+                     "kernel_bytes_per_cell": dom_bytes, "kernel_avg_ms": dom_ms,
+                     "fast_variant": variant,
+                     "stage_bytes_per_cell": list(sb),
+                     "stage_avg_ms": avg,
+                     "step_kernels_GBs": step_gbs,
+                     "step_kernels_frac": step_gbs / HBM_PEAK_GBS,
+                     "frac_at_1088B_per_cell_update": value * BYTES_PER_CELL_STEP_DEF / 1e9 / HBM_PEAK_GBS},
     }
     sim.close()
     if not args.no_cpu:

@@ -382,7 +382,8 @@ StageArgs stage_args(const cosenu_ctx *c, int stage)
     r.a[0] = 0.5 * c->d.dt; r.a[1] = 0.5 * c->d.dt; r.a[2] = c->d.dt; r.a[3] = 1.0 / 6.0 * c->d.dt;
     r.dz = c->d.dz; r.dv = c->d.dv; r.ct = c->d.ct; r.st = c->d.st;
     r.pmo = c->p.pmo; r.mu = c->p.mu; r.ko = c->p.ko;
-    return batch_stage_args(r, stage);
+    // the split-lane FAST kernel uses the stage-value form of RK4
+    return batch_stage_args(r, stage, c->p.mode == COSENU_FAST && c->variant == 2);
 }
 
 RhsArgs rhs_args(const cosenu_ctx *c, const double *in, double *out)

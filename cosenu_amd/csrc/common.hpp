@@ -21,17 +21,26 @@ constexpr int kFlagVac = 1, kFlagMat = 2, kFlagColl = 4, kFlagAdvecOff = 8, kFla
 
 __host__ __device__ inline size_t row_index(int z, int f, int VP) { return ((size_t)z * 8 + f) * VP; }
 
-// Everything a stage kernel needs.  Stage s in 1..4 evaluates K = RHS(yin)
-// and applies the reference's RK4 buffer algebra (nuosc.hpp:320-348):
+// Everything a stage kernel needs.  Stage s in 1..4 evaluates K = RHS(yin).
+// Two forms of the RK4 combination (nuosc.hpp:320-348):
+//  classic (STRICT, and the one-lane FAST variant), the reference's algebra:
 //   s=1: acc = K;            yout = y + a*K        (a = 0.5 dt)
 //   s=2: acc = acc + 2.0*K;  yout = y + a*K        (a = 0.5 dt)
 //   s=3: acc = acc + 2.0*K;  yout = y + a*K        (a = dt)
 //   s=4:                     yout = y + a*(K+acc)  (a = dt/6, yout == y)
+//  stage-value form (split-lane FAST variant): no accumulator; the slopes are
+//  recovered from the stage states, K1 = 2(Y2-y)/dt, K2 = 2(Y3-y)/dt,
+//  K3 = (Y4-y)/dt, so
+//   s=1..3: yout = y + a*K   (Y2, Y3, Y4)
+//   s=4:    yout = y + ((Y2-y) + 2(Y3-y) + (Y4-y))/3 + a*K   (a = dt/6)
+//  which reads y, Y2, Y3 pointwise at stage 4 and moves 832 instead of
+//  1088 bytes per cell and step.
 struct StageArgs {
     const double *yin; // stage input Y_s (read with its z-neighbours)
     const double *y;   // state at the start of the step (pointwise)
-    double *acc;       // RK accumulator
+    double *acc;       // classic form: RK accumulator
     double *yout;      // stage output
+    const double *y2, *y3; // stage-value form, stage 4: Y2 and Y3
     const double *vz;  // [VP] velocity of each bin (padding bins: 0)
     const double *Hm;  // [nz] matter potential (MAT only)
     int nz, nvz, VP, flags, bc, scheme;
@@ -49,14 +58,23 @@ struct BatchRun {
     double dz, dv, ct, st, pmo, mu, ko;
 };
 
-// The buffer roles of RK stage `stage` (1..4), as in the single-run driver.
-__host__ __device__ inline StageArgs batch_stage_args(const BatchRun &r, int stage)
+// The buffer roles of RK stage `stage` (1..4).  Classic form: y -> A -> B ->
+// A -> y with the accumulator in `acc`.  Stage-value form: y -> A (Y2) -> B
+// (Y3) -> acc (Y4) -> y, stage 4 reading A and B pointwise.
+__host__ __device__ inline StageArgs batch_stage_args(const BatchRun &r, int stage, bool yform)
 {
     StageArgs a;
     a.y = r.y;
     a.acc = r.acc;
-    a.yin = stage == 1 ? r.y : (stage == 3 ? r.B : r.A);
-    a.yout = stage == 1 ? r.A : (stage == 2 ? r.B : (stage == 3 ? r.A : r.y));
+    a.y2 = r.A;
+    a.y3 = r.B;
+    if (yform) {
+        a.yin = stage == 1 ? r.y : (stage == 2 ? r.A : (stage == 3 ? r.B : r.acc));
+        a.yout = stage == 1 ? r.A : (stage == 2 ? r.B : (stage == 3 ? r.acc : r.y));
+    } else {
+        a.yin = stage == 1 ? r.y : (stage == 3 ? r.B : r.A);
+        a.yout = stage == 1 ? r.A : (stage == 2 ? r.B : (stage == 3 ? r.A : r.y));
+    }
     a.a = r.a[stage - 1];
     a.vz = r.vz;
     a.Hm = r.Hm;

@@ -172,11 +172,11 @@ __global__ __launch_bounds__(256) void cosenu_fast_stage(StageArgs A, int L, int
         const bool cell = j >= j0;
         const size_t q = (size_t)zrow(j) * zs + v;
         double yv[8], av[8];
-        if (cell && STAGE != 0) {
+        if (cell && STAGE >= 2) {
 #pragma unroll
             for (int f = 0; f < 8; f++) {
                 yv[f] = A.y[q + (size_t)f * VP];
-                av[f] = (STAGE >= 2) ? A.acc[q + (size_t)f * VP] : 0.0;
+                av[f] = A.acc[q + (size_t)f * VP];
             }
         }
 
@@ -278,9 +278,9 @@ __global__ __launch_bounds__(256) void cosenu_fast_stage(StageArgs A, int L, int
                     const size_t o = q + (size_t)f * VP;
                     if (STAGE == 0) {
                         A.yout[o] = K[f];
-                    } else if (STAGE == 1) {
+                    } else if (STAGE == 1) { // y is the stage input: window slot 3
                         A.acc[o] = K[f];
-                        A.yout[o] = yv[f] + A.a * K[f];
+                        A.yout[o] = win[f][3] + A.a * K[f];
                     } else if (STAGE == 4) {
                         A.yout[o] = yv[f] + A.a * (K[f] + av[f]);
                     } else {
@@ -469,12 +469,19 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, bo
         load4(j + 5, nxt);
         const bool cell = j >= j0;
         const size_t q = (size_t)zrow(j) * zs + (size_t)(4 * h) * VP + v;
-        double yv[4], av[4];
-        if (cell && STAGE != 0) {
+        // stage-value form: y (not at stage 1, where y is the stage input and
+        // sits in the window), and Y2, Y3 at stage 4
+        double yv[4], y2v[4], y3v[4];
+        if (cell && (STAGE == 2 || STAGE == 4)) {
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                yv[f] = A.y[q + (size_t)f * VP];
+        }
+        if (cell && STAGE == 4) {
 #pragma unroll
             for (int f = 0; f < 4; f++) {
-                yv[f] = A.y[q + (size_t)f * VP];
-                av[f] = (STAGE >= 2) ? A.acc[q + (size_t)f * VP] : 0.0;
+                y2v[f] = A.y2[q + (size_t)f * VP];
+                y3v[f] = A.y3[q + (size_t)f * VP];
             }
         }
         if (coll)
@@ -568,12 +575,12 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, bo
                     if (STAGE == 0) {
                         A.yout[o] = K[f];
                     } else if (STAGE == 1) {
-                        A.acc[o] = K[f];
-                        A.yout[o] = yv[f] + A.a * K[f];
+                        A.yout[o] = win[f][3] + A.a * K[f];
                     } else if (STAGE == 4) {
-                        A.yout[o] = yv[f] + A.a * (K[f] + av[f]);
+                        const double y0 = yv[f];
+                        const double d = ((y2v[f] - y0) + 2.0 * (y3v[f] - y0)) + (win[f][3] - y0);
+                        A.yout[o] = y0 + (d * (1.0 / 3.0) + A.a * K[f]);
                     } else {
-                        A.acc[o] = av[f] + 2.0 * K[f];
                         A.yout[o] = yv[f] + A.a * K[f];
                     }
                 }
@@ -629,7 +636,7 @@ __global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_batch(const 
     const int item = xcd_block_order() * TPB + team;
     const bool active = item < nitems;
     const int2 it = items[active ? item : 0];
-    const StageArgs A = batch_stage_args(runs[it.x], stage);
+    const StageArgs A = batch_stage_args(runs[it.x], stage, true);
     fast2_body<NW, SCHEME, STAGE, false>(A, L, it.y, active, ring[team]);
 }
 

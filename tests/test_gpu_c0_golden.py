@@ -54,8 +54,10 @@ def test_c0_fast_normwise_after_2000_steps(gpu, golden):
     s0, out, d = _run(C.FAST)
     pee = (out[0] / s0[0])[:, ::int(golden["z_stride"])]
     ref = golden["pee_sub"]
-    assert np.max(np.abs(pee - ref)) / np.max(np.abs(ref)) <= 1e-10
-    assert np.sqrt(np.mean((pee - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)) <= 1e-10
+    linf = np.max(np.abs(pee - ref)) / np.max(np.abs(ref))
+    l2 = np.sqrt(np.mean((pee - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
+    print(f"c0 FAST vs reference after 2000 steps: P_ee normwise Linf {linf:.3e}, L2 {l2:.3e}")
+    assert linf <= 1e-10 and l2 <= 1e-10
     names = [str(n) for n in golden["diag_names"]]
     gd = dict(zip(names, golden["diag"].tolist()))
     for k in ("trace", "btrace", "lepton", "Pee", "Pbee", "M0_norm"):

@@ -46,7 +46,8 @@ class _Info(ctypes.Structure):
     _fields_ = [("dz", ctypes.c_double), ("dv", ctypes.c_double), ("dt", ctypes.c_double),
                 ("ct", ctypes.c_double), ("st", ctypes.c_double), ("phy_time", ctypes.c_double),
                 ("steps_done", ctypes.c_longlong), ("device_bytes", ctypes.c_size_t),
-                ("z_chunk", ctypes.c_int), ("fast_variant", ctypes.c_int)]
+                ("z_chunk", ctypes.c_int), ("fast_variant", ctypes.c_int),
+                ("range_teams", ctypes.c_int)]
 
 
 class _BatchInfo(ctypes.Structure):

@@ -293,6 +293,15 @@ void tune_store(int VP, int scheme, int nz, TuneVal v)
             << kv.second.variant << " " << kv.second.chunk << " " << kv.second.ms << "\n";
 }
 
+// Contiguous-range mode: `mult` times the resident teams, at most one team per
+// 8 z cells.
+int range_teams(int nz, int VP, int scheme, int bc, int mult)
+{
+    long t = (long)fast_resident_teams(VP, scheme, bc) * mult;
+    t = std::min<long>(t, nz / 8);
+    return (int)std::max<long>(t, 1);
+}
+
 // Default z chunk: ~4 wavefronts per SIMD over 1024 SIMDs, within [8, 256],
 // rounded down to a divisor of nz.
 int default_chunk(int nz, int VP)
@@ -324,6 +333,7 @@ struct cosenu_ctx {
     int npart = 0;
     bool have_profile = false, have_state0 = false;
     int z_chunk = 0, variant = 2;
+    int nteams = 0; // > 0: split-lane kernel in contiguous-range mode with this many teams
     View view;
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -401,7 +411,7 @@ hipError_t launch_stage(const cosenu_ctx *c, int stage, hipStream_t s)
 {
     StageArgs a = stage_args(c, stage);
     if (c->p.mode == COSENU_FAST)
-        return launch_stage_fast(a, stage, c->z_chunk, c->variant, s);
+        return launch_stage_fast(a, stage, c->z_chunk, c->variant, c->nteams, s);
     return launch_stage_strict(a, stage, s);
 }
 
@@ -485,21 +495,32 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
     c->VP = ((p->nvz + 63) / 64) * 64;
     const size_t n_state = (size_t)c->nz * 8 * c->VP;
     if (p->mode == COSENU_FAST) {
+        // z_chunk < 0 in the tune table = contiguous-range mode with that many
+        // times the resident teams
+        int chunk;
         TuneVal tv;
         if (tune_lookup(c->VP, p->scheme, p->nz, &tv)) {
             c->variant = tv.variant;
-            c->z_chunk = tv.chunk;
+            chunk = tv.chunk;
         } else {
             c->variant = 2;
-            c->z_chunk = default_chunk(c->nz, c->VP);
+            chunk = default_chunk(c->nz, c->VP);
         }
         int ev = env_int("COSENU_FAST_VARIANT", 0), ez = env_int("COSENU_ZCHUNK", 0);
         if (ev == 1 || ev == 2)
             c->variant = ev;
-        if (ez > 0 && ez <= c->nz && c->nz % ez == 0)
-            c->z_chunk = ez;
+        if (ez != 0)
+            chunk = ez;
         if (p->bc == COSENU_OPEN)
             c->variant = 2; // the open boundary exists in the split-lane kernel only
+        if (chunk < 0 && c->variant == 2) {
+            c->nteams = range_teams(c->nz, c->VP, p->scheme, p->bc, -chunk);
+            c->z_chunk = (c->nz + c->nteams - 1) / c->nteams;
+        } else {
+            if (chunk <= 0 || chunk > c->nz || c->nz % chunk)
+                chunk = default_chunk(c->nz, c->VP);
+            c->z_chunk = chunk;
+        }
     }
 
     int rc = 0;
@@ -588,6 +609,7 @@ int cosenu_get_info(cosenu_ctx *c, cosenu_info *info)
     info->device_bytes = c->bytes;
     info->z_chunk = c->z_chunk;
     info->fast_variant = c->p.mode == COSENU_FAST ? c->variant : 0;
+    info->range_teams = c->nteams;
     return 0;
 }
 
@@ -754,7 +776,7 @@ int cosenu_rhs(cosenu_ctx *c, double *const out[8], ptrdiff_t stride, int off)
     HIP_TRY(hipSetDevice(c->p.device));
     RhsArgs r = rhs_args(c, c->y, c->B);
     if (c->p.mode == COSENU_FAST)
-        HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->variant, c->stream));
+        HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->variant, c->nteams, c->stream));
     else
         HIP_TRY(launch_rhs_strict(r, c->stream));
     return view_download_planes(c->view, c->B, out, stride, off, c->stream);
@@ -864,9 +886,8 @@ const char *cosenu_stage_kernel_name(cosenu_ctx *c, int stage)
                                     : stage_strict_kernel_name(stage);
 }
 
-// Times the middle RK stage (stages 2 and 3 share its kernel) for every
-// kernel variant and z chunk that divides nz, on a scratch context of this
-// shape, and records the fastest in the tune table.
+// Times whole RK4 steps for every kernel variant and launch shape on a
+// scratch context of this shape and records the fastest in the tune table.
 int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *ms_best)
 {
     if (!p)
@@ -882,19 +903,29 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = fail(COSENU_EHIP, "cosenu_autotune: hipEventCreate failed");
-    const int cands[] = {8, 16, 32, 64, 128, 256};
+    // chunk lengths, then (split-lane only) contiguous ranges with 1 or 2
+    // times the resident teams, encoded as chunk = -1, -2
+    const int cands[] = {8, 16, 32, 64, 128, 256, -1, -2};
     for (int var = q.bc == COSENU_OPEN ? 2 : 1; var <= 2 && !rc; var++)
         for (int L : cands) {
-            if (rc || L > c->nz || c->nz % L)
+            if (rc || (L > 0 && (L > c->nz || c->nz % L)) || (L < 0 && var == 1))
                 continue;
             c->variant = var;
-            c->z_chunk = L;
-            const int reps = 5;
-            for (int k = 0; k < 2 + reps && !rc; k++) {
-                if (k == 2 && hipEventRecord(e0, c->stream) != hipSuccess)
+            if (L > 0) {
+                c->z_chunk = L;
+                c->nteams = 0;
+            } else {
+                c->nteams = range_teams(c->nz, c->VP, q.scheme, q.bc, -L);
+                c->z_chunk = (c->nz + c->nteams - 1) / c->nteams;
+            }
+            // whole RK4 steps: the variants move different bytes per stage
+            const int reps = 3;
+            for (int k = 0; k < 1 + reps && !rc; k++) {
+                if (k == 1 && hipEventRecord(e0, c->stream) != hipSuccess)
                     rc = fail(COSENU_EHIP, "cosenu_autotune: event record");
-                if (!rc && launch_stage(c, 2, c->stream) != hipSuccess)
-                    rc = fail(COSENU_EHIP, "cosenu_autotune: launch failed");
+                for (int st = 1; st <= 4 && !rc; st++)
+                    if (launch_stage(c, st, c->stream) != hipSuccess)
+                        rc = fail(COSENU_EHIP, "cosenu_autotune: launch failed");
             }
             if (rc)
                 break;
@@ -918,7 +949,7 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     cosenu_destroy(c);
     if (rc)
         return rc;
-    tune_store(((q.nvz + 63) / 64) * 64, q.scheme, q.nz, {bv, bl, best});
+    tune_store(((q.nvz + 63) / 64) * 64, q.scheme, q.nz, {bv, bl, best}); // ms per step
     if (variant)
         *variant = bv;
     if (z_chunk)

@@ -99,8 +99,13 @@ const char *stage_strict_kernel_name(int stage);
 // FAST mode: returns hipErrorInvalidValue for unsupported shapes.  variant 1
 // = one lane per velocity bin (8 fields per lane), variant 2 = two lanes per
 // bin (nu and nubar halves).
-hipError_t launch_stage_fast(const StageArgs &a, int stage, int z_chunk, int variant, hipStream_t s);
-hipError_t launch_rhs_fast(const RhsArgs &a, int z_chunk, int variant, hipStream_t s);
+// nteams > 0 (split-lane variant only): nteams teams, each streaming one
+// contiguous z range; otherwise chunks of z_chunk cells.
+hipError_t launch_stage_fast(const StageArgs &a, int stage, int z_chunk, int variant, int nteams,
+                             hipStream_t s);
+hipError_t launch_rhs_fast(const RhsArgs &a, int z_chunk, int variant, int nteams, hipStream_t s);
+// teams of the split-lane kernel resident on the current device at once
+int fast_resident_teams(int VP, int scheme, int bc);
 const char *stage_fast_kernel_name(int VP, int stage, int variant);
 // Batched FAST stage over `nitems` work items (run index, first z cell) of
 // runs with the same VP and scheme; `runs` and `items` are device arrays.

@@ -371,8 +371,8 @@ __device__ inline double flux_dir(const double w[kWin], bool pos)
 // extrapolated linearly one after another, and the flux ghosts follow the
 // same rule (rflux[-1] = rflux[0]; lflux[nz] = 2 lflux[nz-1] - lflux[nz-2]).
 template <int NW, int SCHEME, int STAGE, bool OPENB>
-__device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, bool active,
-                                           double (*tring)[NW][8])
+__device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, int jend,
+                                           bool active, double (*tring)[NW][8])
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wit = wave % NW;
@@ -389,6 +389,9 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, bo
     const int dir = anyn ? (anyp ? 0 : -1) : 1;
     const bool store = active && v < A.nvz;
     const double *in = A.yin + (size_t)(4 * h) * VP + v; // this lane's first field
+    // The team updates cells [j0, jend); the loop below runs L + 1 iterations
+    // in every team of a block (jend - j0 <= L) so that all waves meet at
+    // every barrier.
 
     auto zrow = [&](int m) -> int {
         if (OPENB)
@@ -467,7 +470,7 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, bo
     for (int j = j0 - 1; j < j0 + L; j++) {
         double nxt[4];
         load4(j + 5, nxt);
-        const bool cell = j >= j0;
+        const bool cell = j >= j0 && j < jend;
         const size_t q = (size_t)zrow(j) * zs + (size_t)(4 * h) * VP + v;
         // stage-value form: y (not at stage 1, where y is the stage input and
         // sits in the window), and Y2, Y3 at stage 4
@@ -610,16 +613,27 @@ __device__ inline int xcd_block_order()
     return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
 }
 
+// nteams > 0: team g of nteams updates the contiguous range
+// [g*nz/nteams, (g+1)*nz/nteams) (L = the longest range); nteams == 0: team g
+// updates chunk g of length L (nchunks chunks).
 template <int NW, int SCHEME, int STAGE, bool OPENB>
 __global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_stage(StageArgs A, int L,
-                                                                           int nchunks)
+                                                                           int nchunks, int nteams)
 {
     constexpr int TPB = NW >= 4 ? 1 : 4 / NW; // teams per block
     __shared__ double ring[TPB][8][NW][8];
     const int team = (threadIdx.x >> 6) / NW;
-    const int chunk = xcd_block_order() * TPB + team;
-    const bool active = chunk < nchunks;
-    fast2_body<NW, SCHEME, STAGE, OPENB>(A, L, (active ? chunk : 0) * L, active, ring[team]);
+    const int g = xcd_block_order() * TPB + team;
+    if (nteams > 0) {
+        const bool active = g < nteams;
+        const int j0 = active ? (int)(((long)g * A.nz) / nteams) : 0;
+        const int j1 = active ? (int)(((long)(g + 1) * A.nz) / nteams) : 0;
+        fast2_body<NW, SCHEME, STAGE, OPENB>(A, L, j0, j1, active, ring[team]);
+    } else {
+        const bool active = g < nchunks;
+        const int j0 = (active ? g : 0) * L;
+        fast2_body<NW, SCHEME, STAGE, OPENB>(A, L, j0, j0 + L, active, ring[team]);
+    }
 }
 
 // Batched launch: work item k = (run, first z cell); every item of one launch
@@ -637,35 +651,70 @@ __global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_batch(const 
     const bool active = item < nitems;
     const int2 it = items[active ? item : 0];
     const StageArgs A = batch_stage_args(runs[it.x], stage, true);
-    fast2_body<NW, SCHEME, STAGE, false>(A, L, it.y, active, ring[team]);
+    fast2_body<NW, SCHEME, STAGE, false>(A, L, it.y, it.y + L, active, ring[team]);
 }
 
 template <int NW, int SCHEME, bool OPENB>
-hipError_t launch_nw(const StageArgs &a, int stage, int L, hipStream_t s)
+hipError_t launch_nw(const StageArgs &a, int stage, int L, int nteams, hipStream_t s)
 {
     constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
-    const int nchunks = a.nz / L;
-    dim3 grid((nchunks + TPB - 1) / TPB), block(64 * NW * TPB);
+    int nchunks = 0, work;
+    if (nteams > 0) {
+        L = (a.nz + nteams - 1) / nteams;
+        work = nteams;
+    } else {
+        nchunks = a.nz / L;
+        work = nchunks;
+    }
+    dim3 grid((work + TPB - 1) / TPB), block(64 * NW * TPB);
     switch (stage) {
-    case 0: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 0, OPENB>), grid, block, 0, s, a, L, nchunks); break;
-    case 1: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 1, OPENB>), grid, block, 0, s, a, L, nchunks); break;
+    case 0: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 0, OPENB>), grid, block, 0, s, a, L, nchunks, nteams); break;
+    case 1: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 1, OPENB>), grid, block, 0, s, a, L, nchunks, nteams); break;
     case 2:
     case 3: // stages 2 and 3 run the same code with their own coefficient
-        hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 2, OPENB>), grid, block, 0, s, a, L, nchunks); break;
-    case 4: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 4, OPENB>), grid, block, 0, s, a, L, nchunks); break;
+        hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 2, OPENB>), grid, block, 0, s, a, L, nchunks, nteams); break;
+    case 4: hipLaunchKernelGGL((cosenu_fast2_stage<NW, SCHEME, 4, OPENB>), grid, block, 0, s, a, L, nchunks, nteams); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+// Teams of the split-lane kernel that fit on the device at once (one
+// instantiation stands for all: the stage variants differ by a few VGPRs).
+template <int NW, int SCHEME, bool OPENB>
+int resident_teams_nw()
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
+    int nb = 0, dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&cosenu_fast2_stage<NW, SCHEME, 2, OPENB>),
+            64 * NW * TPB, 0) != hipSuccess)
+        return 0;
+    return nb * prop.multiProcessorCount * TPB;
+}
+
 template <int SCHEME, bool OPENB>
-hipError_t launch_scheme2(const StageArgs &a, int stage, int L, hipStream_t s)
+hipError_t launch_scheme2(const StageArgs &a, int stage, int L, int nteams, hipStream_t s)
 {
     switch (a.VP) {
-    case 64: return launch_nw<2, SCHEME, OPENB>(a, stage, L, s);
-    case 128: return launch_nw<4, SCHEME, OPENB>(a, stage, L, s);
-    case 256: return launch_nw<8, SCHEME, OPENB>(a, stage, L, s);
+    case 64: return launch_nw<2, SCHEME, OPENB>(a, stage, L, nteams, s);
+    case 128: return launch_nw<4, SCHEME, OPENB>(a, stage, L, nteams, s);
+    case 256: return launch_nw<8, SCHEME, OPENB>(a, stage, L, nteams, s);
     default: return hipErrorInvalidValue;
+    }
+}
+
+template <int SCHEME, bool OPENB>
+int resident_teams_scheme(int VP)
+{
+    switch (VP) {
+    case 64: return resident_teams_nw<2, SCHEME, OPENB>();
+    case 128: return resident_teams_nw<4, SCHEME, OPENB>();
+    case 256: return resident_teams_nw<8, SCHEME, OPENB>();
+    default: return 0;
     }
 }
 
@@ -702,20 +751,34 @@ hipError_t launch_scheme(const StageArgs &a, int stage, int L, hipStream_t s)
 
 int fast_supported(int VP, int bc) { return (VP == 64 || VP == 128 || VP == 256) && (bc == 0 || bc == 1); }
 
-hipError_t launch_stage_fast(const StageArgs &a, int stage, int L, int variant, hipStream_t s)
+hipError_t launch_stage_fast(const StageArgs &a, int stage, int L, int variant, int nteams,
+                             hipStream_t s)
 {
-    if (!fast_supported(a.VP, a.bc) || L < 1 || a.nz % L != 0)
+    if (!fast_supported(a.VP, a.bc))
         return hipErrorInvalidValue;
+    if (nteams > 0 && variant != 1) {
+        if (nteams > a.nz)
+            return hipErrorInvalidValue;
+    } else if (L < 1 || a.nz % L != 0) {
+        return hipErrorInvalidValue;
+    }
     if (a.bc == 1) // open boundary: split-lane kernel only
-        return a.scheme == 0 ? launch_scheme2<0, true>(a, stage, L, s)
-                             : launch_scheme2<1, true>(a, stage, L, s);
+        return a.scheme == 0 ? launch_scheme2<0, true>(a, stage, L, nteams, s)
+                             : launch_scheme2<1, true>(a, stage, L, nteams, s);
     if (variant == 1)
         return a.scheme == 0 ? launch_scheme<0>(a, stage, L, s) : launch_scheme<1>(a, stage, L, s);
-    return a.scheme == 0 ? launch_scheme2<0, false>(a, stage, L, s)
-                         : launch_scheme2<1, false>(a, stage, L, s);
+    return a.scheme == 0 ? launch_scheme2<0, false>(a, stage, L, nteams, s)
+                         : launch_scheme2<1, false>(a, stage, L, nteams, s);
 }
 
-hipError_t launch_rhs_fast(const RhsArgs &r, int L, int variant, hipStream_t s)
+int fast_resident_teams(int VP, int scheme, int bc)
+{
+    if (bc == 1)
+        return scheme == 0 ? resident_teams_scheme<0, true>(VP) : resident_teams_scheme<1, true>(VP);
+    return scheme == 0 ? resident_teams_scheme<0, false>(VP) : resident_teams_scheme<1, false>(VP);
+}
+
+hipError_t launch_rhs_fast(const RhsArgs &r, int L, int variant, int nteams, hipStream_t s)
 {
     StageArgs a;
     a.yin = r.in; a.y = nullptr; a.acc = nullptr; a.yout = r.out;
@@ -723,7 +786,7 @@ hipError_t launch_rhs_fast(const RhsArgs &r, int L, int variant, hipStream_t s)
     a.nz = r.nz; a.nvz = r.nvz; a.VP = r.VP; a.flags = r.flags; a.bc = r.bc; a.scheme = r.scheme;
     a.a = 0.0;
     a.dz = r.dz; a.dv = r.dv; a.ct = r.ct; a.st = r.st; a.pmo = r.pmo; a.mu = r.mu; a.ko = r.ko;
-    return launch_stage_fast(a, 0, L, variant, s);
+    return launch_stage_fast(a, 0, L, variant, nteams, s);
 }
 
 namespace {

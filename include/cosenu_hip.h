@@ -83,6 +83,8 @@ typedef struct cosenu_info {
     int z_chunk;         /* FAST mode: z cells per streaming work item */
     int fast_variant;    /* FAST mode: 1 = one lane per velocity bin, 2 = two
                             lanes per bin (nu / nubar halves); 0 in STRICT */
+    int range_teams;     /* FAST mode: > 0 when each of this many teams streams
+                            one contiguous z range (z_chunk = its length) */
 } cosenu_info;
 
 /* Diagnostics (analysis.hpp:3-185) plus conservation checks.  All sums are
@@ -181,8 +183,8 @@ int cosenu_get_timing(cosenu_ctx *ctx, double ms[4], long long counts[4]);
 const char *cosenu_stage_kernel_name(cosenu_ctx *ctx, int stage);
 
 /* ------------------------------------------------------------------------
- * Autotuner (no counterpart in the reference).  Times the FAST middle-stage
- * kernel for each kernel variant and z-chunk length that divides nz on a
+ * Autotuner (no counterpart in the reference).  Times whole FAST RK4 steps
+ * for each kernel variant and z-chunk length that divides nz on a
  * scratch context of p's shape, and records the fastest; contexts created
  * afterwards with the same (nvz rounded to 64, scheme, nz) use it.  When the
  * environment variable COSENU_TUNE_FILE names a file, the table is read from

@@ -135,11 +135,19 @@ def main():
     achieved = dom_bytes * cells / (dom_ms * 1e-3) / 1e9
     step_kernel_ms = sum(avg)
     step_gbs = sum(sb) * cells / (step_kernel_ms * 1e-3) / 1e9
+    # HBM bytes per launch of the dominant kernel, from the rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs recorded under profiles/ (scaled by the same counters on a
+    # plain copy); used only when they were taken with this launch configuration.
+    info = sim.info()
+    launch = {"workload": args.workload, "scheme": args.scheme, "fast_variant": variant,
+              "z_chunk": info["z_chunk"], "range_teams": info["range_teams"]}
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.scheme}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get(dom)
+            rec = json.load(f)
+        if rec.get("config") == launch:
+            traffic = rec.get(dom)
 
     out = {
         "metric": METRIC,
@@ -157,7 +165,8 @@ def main():
         "config": {"workload": f"{wl.name}: {args.scheme.upper()} RK4, nz={p.nz} on "
                                f"[{p.z0:g},{p.z1:g}], nvz={p.nvz}, CFL={p.CFL}, periodic, "
                                f"mu=1, omega=0, FAST mode", "cells": cells,
-                   "z_chunk": sim.info()["z_chunk"], "parallelism": "1 GPU"},
+                   "z_chunk": info["z_chunk"], "range_teams": info["range_teams"],
+                   "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": dom_name, "kernel_stage": dom,

@@ -34,7 +34,10 @@ BYTES_PER_CELL_STEP_DEF = 1088
 #   1 (one lane per bin, classic form): 64+128, 192+128, 192+128, 192+64
 #   2 (two lanes per bin, stage-value form: no accumulator; stage 4 reads
 #      y, Y2, Y3 pointwise): 64+64, 128+64, 128+64, 256+64
-STAGE_BYTES = {1: (192, 320, 320, 256), 2: (128, 192, 192, 320)}
+#   3 (barrier-free, stage-value form) as 2, plus the per-z partial moments
+#      of the input read and of the output written: 2 + 2 bytes per cell
+#      (one 64-byte slot per z row and group of 32 bins, each way)
+STAGE_BYTES = {1: (192, 320, 320, 256), 2: (128, 192, 192, 320), 3: (132, 196, 196, 324)}
 
 
 def cpu_baseline(seconds_target: float = 12.0):

@@ -287,17 +287,19 @@ void tune_store(int VP, int scheme, int nz, TuneVal v)
     if (!f || !*f)
         return;
     std::ofstream out(f, std::ios::trunc);
-    out << "# VP scheme nz variant z_chunk ms_per_middle_stage\n";
+    out << "# VP scheme nz variant z_chunk ms_per_step\n";
     for (auto &kv : g_tune)
         out << kv.first.VP << " " << kv.first.scheme << " " << kv.first.nz << " "
             << kv.second.variant << " " << kv.second.chunk << " " << kv.second.ms << "\n";
 }
 
-// Contiguous-range mode: `mult` times the resident teams, at most one team per
-// 8 z cells.
-int range_teams(int nz, int VP, int scheme, int bc, int mult)
+// Contiguous-range mode: `mult` times the resident teams (one team = the
+// wavefronts covering all bins of a z range), at most one team per 8 z cells.
+int range_teams(int nz, int VP, int scheme, int bc, int variant, int mult)
 {
-    long t = (long)fast_resident_teams(VP, scheme, bc) * mult;
+    long t = variant == 3 ? (long)stream_resident_waves(scheme) / (VP / 32)
+                          : (long)fast_resident_teams(VP, scheme, bc);
+    t *= mult;
     t = std::min<long>(t, nz / 8);
     return (int)std::max<long>(t, 1);
 }
@@ -333,7 +335,9 @@ struct cosenu_ctx {
     int npart = 0;
     bool have_profile = false, have_state0 = false;
     int z_chunk = 0, variant = 2;
-    int nteams = 0; // > 0: split-lane kernel in contiguous-range mode with this many teams
+    int nteams = 0; // > 0: contiguous-range mode with this many teams
+    bool stream_ok = false; // the barrier-free kernel (variant 3) applies
+    double *mom[4] = {nullptr, nullptr, nullptr, nullptr}; // its moments of y, A, B, acc
     View view;
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -368,6 +372,9 @@ void release(cosenu_ctx *c)
     for (double *b : bufs)
         if (b)
             (void)hipFree(b);
+    for (double *b : c->mom)
+        if (b)
+            (void)hipFree(b);
     for (hipEvent_t e : c->ev)
         (void)hipEventDestroy(e);
     if (c->graph)
@@ -392,8 +399,14 @@ StageArgs stage_args(const cosenu_ctx *c, int stage)
     r.a[0] = 0.5 * c->d.dt; r.a[1] = 0.5 * c->d.dt; r.a[2] = c->d.dt; r.a[3] = 1.0 / 6.0 * c->d.dt;
     r.dz = c->d.dz; r.dv = c->d.dv; r.ct = c->d.ct; r.st = c->d.st;
     r.pmo = c->p.pmo; r.mu = c->p.mu; r.ko = c->p.ko;
-    // the split-lane FAST kernel uses the stage-value form of RK4
-    return batch_stage_args(r, stage, c->p.mode == COSENU_FAST && c->variant == 2);
+    // the split-lane and barrier-free FAST kernels use the stage-value form
+    StageArgs a = batch_stage_args(r, stage, c->p.mode == COSENU_FAST && c->variant >= 2);
+    if (c->p.mode == COSENU_FAST && c->variant == 3) {
+        // moments travel with the stage states: y, A (Y2), B (Y3), acc (Y4)
+        a.mom_in = c->mom[stage - 1];
+        a.mom_out = c->mom[stage & 3];
+    }
+    return a;
 }
 
 RhsArgs rhs_args(const cosenu_ctx *c, const double *in, double *out)
@@ -410,6 +423,8 @@ RhsArgs rhs_args(const cosenu_ctx *c, const double *in, double *out)
 hipError_t launch_stage(const cosenu_ctx *c, int stage, hipStream_t s)
 {
     StageArgs a = stage_args(c, stage);
+    if (c->p.mode == COSENU_FAST && c->variant == 3)
+        return launch_stage_stream(a, stage, c->z_chunk, c->nteams, s);
     if (c->p.mode == COSENU_FAST)
         return launch_stage_fast(a, stage, c->z_chunk, c->variant, c->nteams, s);
     return launch_stage_strict(a, stage, s);
@@ -494,6 +509,9 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
     c->nvz = p->nvz;
     c->VP = ((p->nvz + 63) / 64) * 64;
     const size_t n_state = (size_t)c->nz * 8 * c->VP;
+    std::vector<double> vz, Hm;
+    fill_grids(*p, c->d, c->VP, vz, Hm);
+    c->stream_ok = p->mode == COSENU_FAST && stream_supported(c->nvz, c->VP, p->bc, vz.data());
     if (p->mode == COSENU_FAST) {
         // z_chunk < 0 in the tune table = contiguous-range mode with that many
         // times the resident teams
@@ -503,18 +521,21 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             c->variant = tv.variant;
             chunk = tv.chunk;
         } else {
+            // split-lane kernel over contiguous z ranges, one team per resident slot
             c->variant = 2;
-            chunk = default_chunk(c->nz, c->VP);
+            chunk = -1;
         }
         int ev = env_int("COSENU_FAST_VARIANT", 0), ez = env_int("COSENU_ZCHUNK", 0);
-        if (ev == 1 || ev == 2)
+        if (ev >= 1 && ev <= 3)
             c->variant = ev;
         if (ez != 0)
             chunk = ez;
+        if (c->variant == 3 && !c->stream_ok)
+            c->variant = 2; // shapes the barrier-free kernel does not take
         if (p->bc == COSENU_OPEN)
             c->variant = 2; // the open boundary exists in the split-lane kernel only
-        if (chunk < 0 && c->variant == 2) {
-            c->nteams = range_teams(c->nz, c->VP, p->scheme, p->bc, -chunk);
+        if (chunk < 0 && c->variant >= 2) {
+            c->nteams = range_teams(c->nz, c->VP, p->scheme, p->bc, c->variant, -chunk);
             c->z_chunk = (c->nz + c->nteams - 1) / c->nteams;
         } else {
             if (chunk <= 0 || chunk > c->nz || c->nz % chunk)
@@ -536,6 +557,8 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
     if (!rc) chk(dev_alloc(c->bytes, &c->prof, (size_t)c->nz * 2 * c->VP));
     if (!rc) chk(dev_alloc(c->bytes, &c->vz, (size_t)c->VP));
     if (!rc) chk(dev_alloc(c->bytes, &c->Hm, (size_t)c->nz));
+    for (int k = 0; k < 4 && c->stream_ok; k++)
+        if (!rc) chk(dev_alloc(c->bytes, &c->mom[k], (size_t)c->nz * (c->VP / 32) * kMomSlot));
     c->npart = diag_partials_needed(c->nz);
     if (!rc) chk(dev_alloc(c->bytes, &c->diag_part, std::max((size_t)c->npart * kDiagSlots,
                                                              (size_t)c->npart * 4 * c->VP)));
@@ -559,8 +582,6 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             e = hipMemset(b, 0, n_state * sizeof(double));
     if (e == hipSuccess)
         e = hipMemset(c->prof, 0, (size_t)c->nz * 2 * c->VP * sizeof(double));
-    std::vector<double> vz, Hm;
-    fill_grids(*p, c->d, c->VP, vz, Hm);
     if (e == hipSuccess)
         e = hipMemcpy(c->vz, vz.data(), c->VP * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -724,6 +745,10 @@ int cosenu_step(cosenu_ctx *c, int nsteps)
     if (!c || nsteps < 0)
         return fail(COSENU_EINVAL, "cosenu_step: bad argument");
     HIP_TRY(hipSetDevice(c->p.device));
+    if (nsteps > 0 && c->p.mode == COSENU_FAST && c->variant == 3)
+        // the state may have been replaced since the last step: its moments
+        // are rebuilt here, after that every stage 4 leaves them current
+        HIP_TRY(launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->VP, c->stream));
     const int gsteps = env_int("COSENU_GRAPH_STEPS", 16);
     int done = 0;
     if (!c->timing && gsteps > 0 && nsteps >= gsteps) {
@@ -775,10 +800,17 @@ int cosenu_rhs(cosenu_ctx *c, double *const out[8], ptrdiff_t stride, int off)
         return fail(COSENU_EINVAL, "cosenu_rhs: bad layout");
     HIP_TRY(hipSetDevice(c->p.device));
     RhsArgs r = rhs_args(c, c->y, c->B);
-    if (c->p.mode == COSENU_FAST)
+    if (c->p.mode == COSENU_FAST && c->variant == 3) {
+        HIP_TRY(launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->VP, c->stream));
+        StageArgs a = stage_args(c, 1);
+        a.yout = c->B;
+        a.mom_out = nullptr;
+        HIP_TRY(launch_stage_stream(a, 0, c->z_chunk, c->nteams, c->stream));
+    } else if (c->p.mode == COSENU_FAST) {
         HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->variant, c->nteams, c->stream));
-    else
+    } else {
         HIP_TRY(launch_rhs_strict(r, c->stream));
+    }
     return view_download_planes(c->view, c->B, out, stride, off, c->stream);
 }
 
@@ -882,6 +914,8 @@ const char *cosenu_stage_kernel_name(cosenu_ctx *c, int stage)
 {
     if (!c)
         return "";
+    if (c->p.mode == COSENU_FAST && c->variant == 3)
+        return stage_stream_kernel_name(stage);
     return c->p.mode == COSENU_FAST ? stage_fast_kernel_name(c->VP, stage, c->variant)
                                     : stage_strict_kernel_name(stage);
 }
@@ -906,16 +940,22 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     // chunk lengths, then (split-lane only) contiguous ranges with 1 or 2
     // times the resident teams, encoded as chunk = -1, -2
     const int cands[] = {8, 16, 32, 64, 128, 256, -1, -2};
-    for (int var = q.bc == COSENU_OPEN ? 2 : 1; var <= 2 && !rc; var++)
+    for (int var = q.bc == COSENU_OPEN ? 2 : 1; var <= 3 && !rc; var++)
         for (int L : cands) {
-            if (rc || (L > 0 && (L > c->nz || c->nz % L)) || (L < 0 && var == 1))
+            if (rc || (L > 0 && (L > c->nz || c->nz % L)) || (L < 0 && var == 1) ||
+                (var == 3 && !c->stream_ok))
                 continue;
             c->variant = var;
+            if (var == 3 && launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->VP,
+                                                  c->stream) != hipSuccess) {
+                rc = fail(COSENU_EHIP, "cosenu_autotune: launch failed");
+                break;
+            }
             if (L > 0) {
                 c->z_chunk = L;
                 c->nteams = 0;
             } else {
-                c->nteams = range_teams(c->nz, c->VP, q.scheme, q.bc, -L);
+                c->nteams = range_teams(c->nz, c->VP, q.scheme, q.bc, var, -L);
                 c->z_chunk = (c->nz + c->nteams - 1) / c->nteams;
             }
             // whole RK4 steps: the variants move different bytes per stage

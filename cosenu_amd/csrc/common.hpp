@@ -43,10 +43,20 @@ struct StageArgs {
     const double *y2, *y3; // stage-value form, stage 4: Y2 and Y3
     const double *vz;  // [VP] velocity of each bin (padding bins: 0)
     const double *Hm;  // [nz] matter potential (MAT only)
+    // barrier-free FAST kernel: per-z partial moments of the stage input
+    // (read) and of the stage output (written), kMomSlot doubles per (z, bin
+    // group of 32)
+    const double *mom_in;
+    double *mom_out;
     int nz, nvz, VP, flags, bc, scheme;
     double a;          // stage coefficient (see above)
     double dz, dv, ct, st, pmo, mu, ko;
 };
+
+// Partial moments of one bin group of 32 at one z: (S, vS, R, vR, D, vD) in
+// the first 6 of kMomSlot doubles (S = ex_im + bex_im, R = ex_re - bex_re,
+// D = ee - xx - bee + bxx summed over the group's bins, v = bin velocity).
+constexpr int kMomSlot = 8;
 
 // One run of a batched launch: its slices of the packed buffers and its
 // physics constants (the runs of one batch may differ in every field).
@@ -78,6 +88,8 @@ __host__ __device__ inline StageArgs batch_stage_args(const BatchRun &r, int sta
     a.a = r.a[stage - 1];
     a.vz = r.vz;
     a.Hm = r.Hm;
+    a.mom_in = nullptr;
+    a.mom_out = nullptr;
     a.nz = r.nz; a.nvz = r.nvz; a.VP = r.VP; a.flags = r.flags; a.bc = r.bc; a.scheme = r.scheme;
     a.dz = r.dz; a.dv = r.dv; a.ct = r.ct; a.st = r.st; a.pmo = r.pmo; a.mu = r.mu; a.ko = r.ko;
     return a;
@@ -106,6 +118,21 @@ hipError_t launch_stage_fast(const StageArgs &a, int stage, int z_chunk, int var
 hipError_t launch_rhs_fast(const RhsArgs &a, int z_chunk, int variant, int nteams, hipStream_t s);
 // teams of the split-lane kernel resident on the current device at once
 int fast_resident_teams(int VP, int scheme, int bc);
+
+// Barrier-free FAST kernel (variant 3, stage_stream.hip): every wavefront
+// marches its own 32 bins in the upwind direction and takes the per-z moments
+// of its input from a_mom_in, written by the launch that produced the input.
+// Needs nvz a multiple of 64, periodic boundaries and bins of one sign of v
+// in every group of 32 (stream_supported).
+int stream_supported(int nvz, int VP, int bc, const double *vz_host);
+hipError_t launch_stage_stream(const StageArgs &a, int stage, int z_chunk, int nteams,
+                               hipStream_t s);
+// partial moments of state y into mom (kMomSlot doubles per z and bin group)
+hipError_t launch_stream_moments(const double *y, const double *vz, double *mom, int nz, int VP,
+                                 hipStream_t s);
+// waves of the barrier-free kernel resident on the current device at once
+int stream_resident_waves(int scheme);
+const char *stage_stream_kernel_name(int stage);
 const char *stage_fast_kernel_name(int VP, int stage, int variant);
 // Batched FAST stage over `nitems` work items (run index, first z cell) of
 // runs with the same VP and scheme; `runs` and `items` are device arrays.

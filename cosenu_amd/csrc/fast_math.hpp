@@ -1,0 +1,158 @@
+// Device helpers shared by the FAST stage kernels: the upwind WENO7
+// reconstruction, the wavefront reduction of the per-z moments and the
+// XCD-aware block order.
+#pragma once
+
+#include "common.hpp"
+
+namespace cosenu {
+namespace fastk {
+
+__device__ inline double wave_sum(double x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// eps + beta for a smoothness indicator written as a quadratic form in three
+// consecutive differences d1, d2, d3 of its 4-cell stencil:
+//   beta = c11 d1^2 + c12 d1 d2 + c13 d1 d3 + c22 d2^2 + c23 d2 d3 + c33 d3^2.
+__device__ __forceinline__ double eps_beta(double c11, double c12, double c13, double c22,
+                                           double c23, double c33, double eps, double d1,
+                                           double d2, double d3)
+{
+    const double t1 = fma(c13, d3, fma(c12, d2, c11 * d1));
+    const double t2 = fma(c23, d3, c22 * d2);
+    return fma(d1, t1, fma(d2, t2, fma(d3, c33 * d3, eps)));
+}
+
+// Upwind WENO7 value from the upwind-ordered stencil a0..a6 (a3 = the cell
+// on the upwind side of the interface).
+//
+// Same mathematics as rhs_fv.hpp:49-80, evaluated in difference form.  The
+// reference's smoothness indicators are quadratic forms in the cell values
+// whose coefficient rows sum to zero, so each is exactly a quadratic form in
+// the three differences D_i = a(i+1) - a(i) of its stencil (coefficients
+// obtained from the reference's by exact rational substitution); the
+// candidate values are a3 plus a combination of differences with weights
+// (1/12)(9,-4,1), (1/12)(1,6,-1), (1/12)(-1,4,3), (1/12)(3,-10,13).  Both forms
+// are equal in exact arithmetic; this one needs fewer operations and avoids
+// the cancellation of the expanded form (terms of size 1e4 u^2 summing to a
+// beta of size (du)^2).
+//
+// Nonlinear weights w_k = (g_k / q_k) / sum_m (g_m / q_m), q_k = (eps +
+// beta_k)^2, linear weights g = (4, 18, 12, 1)/35.  With r_k = q_k / g_k,
+// w_k = prod_{m != k} r_m / sum_k prod_{m != k} r_m: one division.  r_k is
+// formed as ((eps + beta_k) / sqrt(g_k))^2 with 1/sqrt(g_k) folded into the
+// indicator's coefficients (the common factor 35 cancels).
+__device__ inline double weno_up(double a0, double a1, double a2, double a3, double a4, double a5,
+                                 double a6)
+{
+    constexpr double eps = 1e-6;
+    constexpr double h1 = 0.23570226039551584; // 1/sqrt(18)
+    constexpr double h2 = 0.28867513459481287; // 1/sqrt(12)
+    const double D0 = a1 - a0, D1 = a2 - a1, D2 = a3 - a2, D3 = a4 - a3, D4 = a5 - a4,
+                 D5 = a6 - a5;
+    // (eps + beta_k) / sqrt(g_k); 1/sqrt(4) = 0.5 and 1/sqrt(1) are exact
+    const double t0 = eps_beta(0.5 * 2107, 0.5 * -5188, 0.5 * 1854, 0.5 * 3708, 0.5 * -2788,
+                               0.5 * 547, 0.5 * eps, D3, D4, D5);
+    const double t1 = eps_beta(h1 * 547, h1 * -1428, h1 * 494, h1 * 1468, h1 * -1108, h1 * 267,
+                               h1 * eps, D2, D3, D4);
+    const double t2 = eps_beta(h2 * 267, h2 * -1108, h2 * 494, h2 * 1468, h2 * -1428, h2 * 547,
+                               h2 * eps, D1, D2, D3);
+    const double t3 = eps_beta(547, -2788, 1854, 3708, -5188, 2107, eps, D0, D1, D2);
+    // 12 x (candidate value - a3)
+    const double c0 = fma(9.0, D3, fma(-4.0, D4, D5));
+    const double c1 = fma(6.0, D3, D2 - D4);
+    const double c2 = fma(4.0, D2, fma(3.0, D3, -D1));
+    const double c3 = fma(3.0, D0, fma(-10.0, D1, 13.0 * D2));
+    const double r0 = t0 * t0, r1 = t1 * t1, r2 = t2 * t2, r3 = t3 * t3;
+    const double p01 = r0 * r1, p23 = r2 * r3;
+    const double e0 = r1 * p23, e1 = r0 * p23, e2 = r3 * p01, e3 = r2 * p01;
+    const double num = e0 * c0 + e1 * c1 + e2 * c2 + e3 * c3;
+    const double den = (e0 + e1) + (e2 + e3);
+    // num/den: hardware reciprocal refined by one Newton step (den is a
+    // positive normal number).  The quotient is a correction to a3 of the
+    // size of the local differences, so its last-bit error is far below an
+    // ulp of the result.
+    double r = __builtin_amdgcn_rcp(den);
+    r = fma(r, fma(-den, r, 1.0), r);
+    return fma(num * r, 1.0 / 12.0, a3);
+}
+
+// Cross-lane moves of a 64-bit value, one 32-bit half at a time.
+//   swap32(a, b): a <- [a(0..31) | b(0..31)], b <- [a(32..63) | b(32..63)]
+//   swap16(a, b): rows of 16 lanes, a <- [a0 b0 a2 b2], b <- [a1 b1 a3 b3]
+__device__ inline void swap32(double &a, double &b)
+{
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a),
+                                                     (unsigned)__double2loint(b), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a),
+                                                     (unsigned)__double2hiint(b), false, false);
+    a = __hiloint2double((int)hi[0], (int)lo[0]);
+    b = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+__device__ inline void swap16(double &a, double &b)
+{
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(a),
+                                                     (unsigned)__double2loint(b), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(a),
+                                                     (unsigned)__double2hiint(b), false, false);
+    a = __hiloint2double((int)hi[0], (int)lo[0]);
+    b = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+// x from another lane of the same row of 16 (DPP control CTRL).
+template <int CTRL>
+__device__ inline double dpp(double x)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum of 6 per-lane values m[0..5] over the 64 lanes.  Each level halves the
+// lanes a value is spread over and, for the first three, pairs two values in
+// one register: lane swaps across the halves (32) and rows (16) of the
+// wavefront, then row-local DPP moves (rotate by 8, half-row mirror, two quad
+// permutations).  On return lane l holds the total of value
+// kMomentLane[l >> 3] (values 6 and 7 mark duplicates, not to be stored).
+constexpr unsigned kMomentLane = 0x74537120u; // nibble g: value in lanes 8g .. 8g+7
+
+__device__ inline double wave_sum6(const double m[6], int lane)
+{
+    double p0 = m[0], p1 = m[1], p2 = m[2], p3 = m[3], p4 = m[4], p5 = m[5];
+    swap32(p0, p3);
+    p0 += p3; // lanes 0-31: value 0, lanes 32-63: value 3 (16 lanes apart still split)
+    swap32(p1, p4);
+    p1 += p4; // values 1 / 4
+    swap32(p2, p5);
+    p2 += p5; // values 2 / 5
+    swap16(p0, p1);
+    p0 += p1; // rows: values 0, 1, 3, 4
+    double p2b = p2;
+    swap16(p2, p2b);
+    p2 += p2b; // rows: values 2, 2, 5, 5
+    const bool hi8 = (lane >> 3) & 1;
+    double x = (hi8 ? p2 : p0) + dpp<0x128>(hi8 ? p0 : p2); // row_ror:8
+    x += dpp<0x141>(x);                                      // row_half_mirror
+    x += dpp<0xB1>(x);                                       // quad_perm [1,0,3,2]
+    x += dpp<0x4E>(x);                                       // quad_perm [2,3,0,1]
+    return x;
+}
+
+// XCD-aware order: blocks are dealt round-robin to the 8 XCDs, so give each
+// XCD a contiguous run of work (neighbouring chunks share halo rows in that
+// XCD's L2).
+__device__ inline int xcd_block_order()
+{
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = nb / 8, rem = nb % 8, x = b % 8, k = b / 8;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
+}
+
+} // namespace fastk
+} // namespace cosenu

@@ -26,62 +26,16 @@
 // The 6 moments of z row j+4 are reduced across the wavefront when that row
 // enters the window and parked in an LDS ring of 8 rows; cell j reads them 4
 // iterations later, so the cross-wave exchange never waits on fresh data.
-#include "common.hpp"
+#include "fast_math.hpp"
 #include <cstdio>
 #include <cstdlib>
 
 namespace cosenu {
 namespace {
 
+using namespace fastk;
+
 constexpr int kWin = 8; // u(j-3) .. u(j+4)
-
-__device__ inline double wave_sum(double x)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1)
-        x += __shfl_xor(x, o, 64);
-    return x;
-}
-
-// Upwind WENO7 value from the upwind-ordered stencil a0..a6 (a3 = the cell
-// on the upwind side of the interface).
-__device__ inline double weno_up(double a0, double a1, double a2, double a3, double a4, double a5,
-                                 double a6)
-{
-    // Smoothness indicators (rhs_fv.hpp:49-52 with u[k] = a(3+k)).
-    double b0 = a3 * (2107.0 * a3 - 9402.0 * a4 + 7042.0 * a5 - 1854.0 * a6)
-              + a4 * (11003.0 * a4 - 17246.0 * a5 + 4642.0 * a6)
-              + a5 * (7043.0 * a5 - 3882.0 * a6) + 547.0 * a6 * a6;
-    double b1 = a2 * (547.0 * a2 - 2522.0 * a3 + 1922.0 * a4 - 494.0 * a5)
-              + a3 * (3443.0 * a3 - 5966.0 * a4 + 1602.0 * a5)
-              + a4 * (2843.0 * a4 - 1642.0 * a5) + 267.0 * a5 * a5;
-    double b2 = a1 * (267.0 * a1 - 1642.0 * a2 + 1602.0 * a3 - 494.0 * a4)
-              + a2 * (2843.0 * a2 - 5966.0 * a3 + 1922.0 * a4)
-              + a3 * (3443.0 * a3 - 2522.0 * a4) + 547.0 * a4 * a4;
-    double b3 = a0 * (547.0 * a0 - 3882.0 * a1 + 4642.0 * a2 - 1854.0 * a3)
-              + a1 * (7043.0 * a1 - 17246.0 * a2 + 7042.0 * a3)
-              + a2 * (11003.0 * a2 - 9402.0 * a3) + 2107.0 * a3 * a3;
-    // Candidate polynomials at the interface (rhs_fv.hpp:67-70).
-    double c0 = (1. / 4.) * a3 + (13. / 12.) * a4 - (5. / 12.) * a5 + (1. / 12.) * a6;
-    double c1 = (-1. / 12.) * a2 + (7. / 12.) * a3 + (7. / 12.) * a4 - (1. / 12.) * a5;
-    double c2 = (1. / 12.) * a1 - (5. / 12.) * a2 + (13. / 12.) * a3 + (1. / 4.) * a4;
-    double c3 = (-1. / 4.) * a0 + (13. / 12.) * a1 - (23. / 12.) * a2 + (25. / 12.) * a3;
-    const double eps = 1e-6;
-    double q0 = eps + b0, q1 = eps + b1, q2 = eps + b2, q3 = eps + b3;
-    q0 *= q0; q1 *= q1; q2 *= q2; q3 *= q3;
-    double p01 = q0 * q1, p23 = q2 * q3;
-    // linear weights (4, 18, 12, 1)/35; the 1/35 cancels
-    double e0 = 4.0 * (q1 * p23), e1 = 18.0 * (q0 * p23), e2 = 12.0 * (p01 * q3), e3 = p01 * q2;
-    double num = e0 * c0 + e1 * c1 + e2 * c2 + e3 * c3;
-    double den = (e0 + e1) + (e2 + e3);
-    // num/den: hardware reciprocal, one Newton step, then a residual
-    // correction of the quotient (within an ulp of the IEEE quotient; den is
-    // a positive normal number, at least (1e-6)^6).
-    double r = __builtin_amdgcn_rcp(den);
-    r = fma(r, fma(-den, r, 1.0), r);
-    const double qt = num * r;
-    return fma(r, fma(-den, qt, num), qt);
-}
 
 template <int W, int SCHEME, int STAGE>
 __global__ __launch_bounds__(256) void cosenu_fast_stage(StageArgs A, int L, int nchunks)
@@ -322,37 +276,6 @@ __global__ __launch_bounds__(256) void cosenu_fast_stage(StageArgs A, int L, int
 //   K_s =  2 ct r + st (p - q) - sigma h r + sigma (p - q) R - sigma r D
 // with S, R, D the collective integrals mu dv (F0 - v F1) and h = Hm(z).
 
-// Sum of 6 per-lane values over the 64 lanes: a butterfly that halves the
-// number of values carried at each of the first three levels.  Returns the
-// total of value ((l>>5)&1)*4 + ((l>>4)&1)*2 + ((l>>3)&1) in lane l (values
-// 6 and 7 are zero padding).
-__device__ inline double wave_sum6(const double m6[6], int lane)
-{
-    double m[8] = {m6[0], m6[1], m6[2], m6[3], m6[4], m6[5], 0.0, 0.0};
-    const bool b5 = (lane >> 5) & 1, b4 = (lane >> 4) & 1, b3 = (lane >> 3) & 1;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        double send = b5 ? m[k] : m[k + 4];
-        double keep = b5 ? m[k + 4] : m[k];
-        m[k] = keep + __shfl_xor(send, 32, 64);
-    }
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        double send = b4 ? m[k] : m[k + 2];
-        double keep = b4 ? m[k + 2] : m[k];
-        m[k] = keep + __shfl_xor(send, 16, 64);
-    }
-    {
-        double send = b3 ? m[0] : m[1];
-        double keep = b3 ? m[1] : m[0];
-        m[0] = keep + __shfl_xor(send, 8, 64);
-    }
-    m[0] += __shfl_xor(m[0], 4, 64);
-    m[0] += __shfl_xor(m[0], 2, 64);
-    m[0] += __shfl_xor(m[0], 1, 64);
-    return m[0];
-}
-
 template <int DIR>
 __device__ inline double flux_dir(const double w[kWin], bool pos)
 {
@@ -381,6 +304,9 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, in
     const int VP = A.VP, nz = A.nz;
     const size_t zs = (size_t)8 * VP;
     const double sig = h ? -1.0 : 1.0;
+    // z positions are the same in every lane of a wavefront: keep them scalar
+    j0 = __builtin_amdgcn_readfirstlane(j0);
+    jend = __builtin_amdgcn_readfirstlane(jend);
 
     const double vi = A.vz[v];
     const bool pos = vi > 0.0;
@@ -396,8 +322,10 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, in
     auto zrow = [&](int m) -> int {
         if (OPENB)
             return m;
-        int w = m % nz;
-        return w < 0 ? w + nz : w;
+        int w = m < 0 ? m + nz : m; // m >= -4 > -nz
+        while (w >= nz)
+            w -= nz;
+        return w;
     };
     // the 4 fields of this lane at z row m (m is wave-uniform)
     auto load4 = [&](int m, double o[4]) {
@@ -445,7 +373,7 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, in
         const double D = live ? sig * (win[0][kk] - win[1][kk]) : 0.0;
         const double m6[6] = {S, vi * S, R, vi * R, D, vi * D};
         const double t = wave_sum6(m6, lane);
-        const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+        const int idx = (kMomentLane >> (4 * (lane >> 3))) & 15;
         if ((lane & 7) == 0 && idx < 6)
             tring[zabs & 7][wit][idx] = t;
     };
@@ -601,16 +529,6 @@ __device__ __forceinline__ void fast2_body(const StageArgs &A, int L, int j0, in
             Fprev[f] = F[f];
         }
     }
-}
-
-// XCD-aware order: blocks are dealt round-robin to the 8 XCDs, so give each
-// XCD a contiguous run of work (neighbouring chunks share halo rows in that
-// XCD's L2).
-__device__ inline int xcd_block_order()
-{
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int per = nb / 8, rem = nb % 8, x = b % 8, k = b / 8;
-    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
 }
 
 // nteams > 0: team g of nteams updates the contiguous range
@@ -782,6 +700,7 @@ hipError_t launch_rhs_fast(const RhsArgs &r, int L, int variant, int nteams, hip
 {
     StageArgs a;
     a.yin = r.in; a.y = nullptr; a.acc = nullptr; a.yout = r.out;
+    a.y2 = nullptr; a.y3 = nullptr; a.mom_in = nullptr; a.mom_out = nullptr;
     a.vz = r.vz; a.Hm = r.Hm;
     a.nz = r.nz; a.nvz = r.nvz; a.VP = r.VP; a.flags = r.flags; a.bc = r.bc; a.scheme = r.scheme;
     a.a = 0.0;

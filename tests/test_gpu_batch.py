@@ -95,8 +95,8 @@ def test_autotune_records_and_applies(gpu, tmp_path, monkeypatch):
     monkeypatch.delenv("COSENU_ZCHUNK", raising=False)
     p = C.RunParams(nz=4096, nvz=128, mode=C.FAST)
     v, L, ms = C.autotune(p)
-    assert v in (1, 2) and ms > 0
-    assert (L > 0 and 4096 % L == 0) or (v == 2 and L in (-1, -2))
+    assert v in (1, 2, 3) and ms > 0
+    assert (L > 0 and 4096 % L == 0) or (v >= 2 and L in (-1, -2))
     rows = [l.split() for l in table.read_text().splitlines() if not l.startswith("#")]
     assert ["128", "0", "4096", str(v), str(L)] == rows[-1][:5]
     with C.Simulation(p) as sim:
@@ -108,11 +108,12 @@ def test_autotune_records_and_applies(gpu, tmp_path, monkeypatch):
         assert info["range_teams"] > 0
 
 
+@pytest.mark.parametrize("variant", [2, 3])
 @pytest.mark.parametrize("mult", [1, 2])
-def test_range_launch_equals_chunked_launch(gpu, monkeypatch, mult):
+def test_range_launch_equals_chunked_launch(gpu, monkeypatch, mult, variant):
     """The contiguous-range launch computes the same cells with the same
     arithmetic as the chunked one: bit-identical states."""
-    monkeypatch.setenv("COSENU_FAST_VARIANT", "2")
+    monkeypatch.setenv("COSENU_FAST_VARIANT", str(variant))
     p = C.RunParams(nz=6000, nvz=128, z0=-300, z1=300, mode=C.FAST)
     _, _, s0 = seeded_ic(p, 0.6, 0.5, 0.9, seed=9, amp=1e-3)
     outs = []
@@ -124,4 +125,5 @@ def test_range_launch_equals_chunked_launch(gpu, monkeypatch, mult):
             sim.step(20)
             outs.append(sim.download())
         assert (info["range_teams"] > 0) == (chunk.startswith("-"))
+        assert info["fast_variant"] == variant
     assert np.array_equal(outs[0], outs[1])

@@ -297,8 +297,9 @@ void tune_store(int VP, int scheme, int nz, TuneVal v)
 // wavefronts covering all bins of a z range), at most one team per 8 z cells.
 int range_teams(int nz, int VP, int scheme, int bc, int variant, int mult)
 {
-    long t = variant == 3 ? (long)stream_resident_waves(scheme) / (VP / 32)
-                          : (long)fast_resident_teams(VP, scheme, bc);
+    long t = variant == 3   ? (long)stream_resident_waves(scheme) / (VP / 32)
+             : variant == 4 ? (long)pair_resident_teams(VP, scheme)
+                            : (long)fast_resident_teams(VP, scheme, bc);
     t *= mult;
     t = std::min<long>(t, nz / 8);
     return (int)std::max<long>(t, 1);
@@ -346,6 +347,7 @@ struct cosenu_ctx {
     long long cnt[4] = {0, 0, 0, 0};
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    const double *graph_y = nullptr; // the state buffer the graph was captured with
     hipEvent_t snap_ready = nullptr, snap_done = nullptr;
     bool snap_pending = false;
 };
@@ -430,23 +432,46 @@ hipError_t launch_stage(const cosenu_ctx *c, int stage, hipStream_t s)
     return launch_stage_strict(a, stage, s);
 }
 
+bool two_stage_launches(const cosenu_ctx *c) { return c->p.mode == COSENU_FAST && c->variant == 4; }
+
+// Stages 1+2 (pair 0) or 3+4 (pair 1) in one launch; the new state of pair 1
+// goes to the spare state buffer (acc), which becomes y after the step.
+hipError_t launch_pair(const cosenu_ctx *c, int pair, hipStream_t s)
+{
+    StageArgs a1 = stage_args(c, pair == 0 ? 1 : 3), a2 = stage_args(c, pair == 0 ? 2 : 4);
+    if (pair == 1)
+        a2.yout = c->acc;
+    return launch_stage_pair(a1, a2, pair, c->z_chunk, c->nteams, s);
+}
+
+void swap_state(cosenu_ctx *c)
+{
+    std::swap(c->y, c->acc);
+    c->view.y = c->y;
+}
+
+// The launches of one RK4 step; with timing, events around each launch
+// (timing slot k = launch k of the step).
 int enqueue_step(cosenu_ctx *c, hipStream_t s)
 {
-    for (int st = 1; st <= 4; st++) {
+    const bool pairs = two_stage_launches(c);
+    for (int k = 0; k < (pairs ? 2 : 4); k++) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (c->timing) {
             HIP_TRY(hipEventCreate(&e0));
             HIP_TRY(hipEventCreate(&e1));
             HIP_TRY(hipEventRecord(e0, s));
         }
-        HIP_TRY(launch_stage(c, st, s));
+        HIP_TRY(pairs ? launch_pair(c, k, s) : launch_stage(c, k + 1, s));
         if (c->timing) {
             HIP_TRY(hipEventRecord(e1, s));
             c->ev.push_back(e0);
             c->ev.push_back(e1);
-            c->ev_stage.push_back(st - 1);
+            c->ev_stage.push_back(k);
         }
     }
+    if (pairs)
+        swap_state(c);
     return 0;
 }
 
@@ -526,12 +551,14 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             chunk = -1;
         }
         int ev = env_int("COSENU_FAST_VARIANT", 0), ez = env_int("COSENU_ZCHUNK", 0);
-        if (ev >= 1 && ev <= 3)
+        if (ev >= 1 && ev <= 4)
             c->variant = ev;
         if (ez != 0)
             chunk = ez;
         if (c->variant == 3 && !c->stream_ok)
             c->variant = 2; // shapes the barrier-free kernel does not take
+        if (c->variant == 4 && (!pair_supported(c->VP, p->bc) || c->nz < 16))
+            c->variant = 2;
         if (p->bc == COSENU_OPEN)
             c->variant = 2; // the open boundary exists in the split-lane kernel only
         if (chunk < 0 && c->variant >= 2) {
@@ -749,22 +776,20 @@ int cosenu_step(cosenu_ctx *c, int nsteps)
         // the state may have been replaced since the last step: its moments
         // are rebuilt here, after that every stage 4 leaves them current
         HIP_TRY(launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->VP, c->stream));
-    const int gsteps = env_int("COSENU_GRAPH_STEPS", 16);
+    // two-stage launches alternate the state buffers every step: a graph of
+    // an even number of steps ends on the buffer it started from
+    int gsteps = env_int("COSENU_GRAPH_STEPS", 16);
+    if (two_stage_launches(c))
+        gsteps &= ~1;
     int done = 0;
     if (!c->timing && gsteps > 0 && nsteps >= gsteps) {
-        if (!c->graph || c->graph_steps != gsteps) {
+        if (!c->graph || c->graph_steps != gsteps || c->graph_y != c->y) {
             if (c->graph) {
                 (void)hipGraphExecDestroy(c->graph);
                 c->graph = nullptr;
             }
-            auto one = [c]() -> int {
-                for (int st = 1; st <= 4; st++) {
-                    hipError_t e = launch_stage(c, st, c->stream);
-                    if (e != hipSuccess)
-                        return fail(COSENU_EHIP, std::string("capture launch: ") + hipGetErrorString(e));
-                }
-                return 0;
-            };
+            c->graph_y = c->y;
+            auto one = [c]() -> int { return enqueue_step(c, c->stream); };
             RC_TRY(capture_graph(c->stream, gsteps, one, &c->graph));
             c->graph_steps = gsteps;
         }
@@ -806,8 +831,9 @@ int cosenu_rhs(cosenu_ctx *c, double *const out[8], ptrdiff_t stride, int off)
         a.yout = c->B;
         a.mom_out = nullptr;
         HIP_TRY(launch_stage_stream(a, 0, c->z_chunk, c->nteams, c->stream));
-    } else if (c->p.mode == COSENU_FAST) {
-        HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->variant, c->nteams, c->stream));
+    } else if (c->p.mode == COSENU_FAST) { // variant 4 evaluates single RHSs with variant 2
+        HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->variant == 4 ? 2 : c->variant, c->nteams,
+                                c->stream));
     } else {
         HIP_TRY(launch_rhs_strict(r, c->stream));
     }
@@ -916,6 +942,8 @@ const char *cosenu_stage_kernel_name(cosenu_ctx *c, int stage)
         return "";
     if (c->p.mode == COSENU_FAST && c->variant == 3)
         return stage_stream_kernel_name(stage);
+    if (two_stage_launches(c))
+        return stage < 2 ? stage_pair_kernel_name(stage) : "";
     return c->p.mode == COSENU_FAST ? stage_fast_kernel_name(c->VP, stage, c->variant)
                                     : stage_strict_kernel_name(stage);
 }
@@ -940,10 +968,11 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     // chunk lengths, then (split-lane only) contiguous ranges with 1 or 2
     // times the resident teams, encoded as chunk = -1, -2
     const int cands[] = {8, 16, 32, 64, 128, 256, -1, -2};
-    for (int var = q.bc == COSENU_OPEN ? 2 : 1; var <= 3 && !rc; var++)
+    for (int var = q.bc == COSENU_OPEN ? 2 : 1; var <= 4 && !rc; var++)
         for (int L : cands) {
             if (rc || (L > 0 && (L > c->nz || c->nz % L)) || (L < 0 && var == 1) ||
-                (var == 3 && !c->stream_ok))
+                (var == 3 && !c->stream_ok) ||
+                (var == 4 && (!pair_supported(c->VP, q.bc) || c->nz < 16)))
                 continue;
             c->variant = var;
             if (var == 3 && launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->VP,
@@ -963,9 +992,8 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
             for (int k = 0; k < 1 + reps && !rc; k++) {
                 if (k == 1 && hipEventRecord(e0, c->stream) != hipSuccess)
                     rc = fail(COSENU_EHIP, "cosenu_autotune: event record");
-                for (int st = 1; st <= 4 && !rc; st++)
-                    if (launch_stage(c, st, c->stream) != hipSuccess)
-                        rc = fail(COSENU_EHIP, "cosenu_autotune: launch failed");
+                if (!rc)
+                    rc = enqueue_step(c, c->stream);
             }
             if (rc)
                 break;
@@ -1076,9 +1104,17 @@ int group_chunk(const cosenu_batch *b, const Group &g)
     for (int r : g.runs)
         gcd = std::gcd(gcd, b->P[r].nz);
     int want = env_int("COSENU_BATCH_ZCHUNK", 0);
+    TuneVal tv;
+    if (want <= 0 && tune_lookup(g.VP, g.scheme, -1, &tv))
+        want = tv.chunk;
     if (want <= 0) {
-        TuneVal tv;
-        want = tune_lookup(g.VP, g.scheme, -1, &tv) ? tv.chunk : 32;
+        // about two work items per resident team, chunks of 8 to 32 cells:
+        // a few runs still spread over the whole device
+        long rows = 0;
+        for (int r : g.runs)
+            rows += b->P[r].nz;
+        const long target = 2L * std::max(1, fast_resident_teams(g.VP, g.scheme, 0));
+        want = (int)std::max(8L, std::min(32L, rows / target));
     }
     int L = std::min(want, gcd);
     while (L > 1 && gcd % L)

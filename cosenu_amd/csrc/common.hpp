@@ -133,6 +133,17 @@ hipError_t launch_stream_moments(const double *y, const double *vz, double *mom,
 // waves of the barrier-free kernel resident on the current device at once
 int stream_resident_waves(int scheme);
 const char *stage_stream_kernel_name(int stage);
+
+// Two RK4 stages per launch (variant 4, stage_pair.hip): pair 0 runs stages
+// 1 and 2 (a1, a2 = their stage arguments), pair 1 stages 3 and 4; the
+// second stage's input never goes through HBM.  a2.yout of pair 1 must not
+// alias a1.y (the new state goes to a second buffer).  Periodic boundaries,
+// VP in {64, 128, 256}.
+int pair_supported(int VP, int bc);
+hipError_t launch_stage_pair(const StageArgs &a1, const StageArgs &a2, int pair, int z_chunk,
+                             int nteams, hipStream_t s);
+int pair_resident_teams(int VP, int scheme);
+const char *stage_pair_kernel_name(int pair);
 const char *stage_fast_kernel_name(int VP, int stage, int variant);
 // Batched FAST stage over `nitems` work items (run index, first z cell) of
 // runs with the same VP and scheme; `runs` and `items` are device arrays.

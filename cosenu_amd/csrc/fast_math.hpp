@@ -82,6 +82,20 @@ __device__ inline double weno_up(double a0, double a1, double a2, double a3, dou
     return fma(num * r, 1.0 / 12.0, a3);
 }
 
+// Upwind face value from a window w[k] = u(j-3+k), k = 0..7: DIR > 0 the
+// left-biased value at j+1/2 from u(j-3..j+3), DIR < 0 the right-biased one
+// from the mirrored stencil u(j+4..j-2), DIR == 0 per lane by `pos`.
+template <int DIR>
+__device__ inline double flux_dir(const double w[8], bool pos)
+{
+    if (DIR > 0)
+        return weno_up(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
+    if (DIR < 0)
+        return weno_up(w[7], w[6], w[5], w[4], w[3], w[2], w[1]);
+    return weno_up(pos ? w[0] : w[7], pos ? w[1] : w[6], pos ? w[2] : w[5], pos ? w[3] : w[4],
+                   pos ? w[4] : w[3], pos ? w[5] : w[2], pos ? w[6] : w[1]);
+}
+
 // Cross-lane moves of a 64-bit value, one 32-bit half at a time.
 //   swap32(a, b): a <- [a(0..31) | b(0..31)], b <- [a(32..63) | b(32..63)]
 //   swap16(a, b): rows of 16 lanes, a <- [a0 b0 a2 b2], b <- [a1 b1 a3 b3]

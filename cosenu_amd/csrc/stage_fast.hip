@@ -276,17 +276,6 @@ __global__ __launch_bounds__(256) void cosenu_fast_stage(StageArgs A, int L, int
 //   K_s =  2 ct r + st (p - q) - sigma h r + sigma (p - q) R - sigma r D
 // with S, R, D the collective integrals mu dv (F0 - v F1) and h = Hm(z).
 
-template <int DIR>
-__device__ inline double flux_dir(const double w[kWin], bool pos)
-{
-    if (DIR > 0)
-        return weno_up(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
-    if (DIR < 0)
-        return weno_up(w[7], w[6], w[5], w[4], w[3], w[2], w[1]);
-    return weno_up(pos ? w[0] : w[7], pos ? w[1] : w[6], pos ? w[2] : w[5], pos ? w[3] : w[4],
-                   pos ? w[4] : w[3], pos ? w[5] : w[2], pos ? w[6] : w[1]);
-}
-
 // One team's march over the chunk [j0, j0+L) of the run described by A.
 // `tring` is the team's LDS ring.  Teams of a block must call this with the
 // same L (they meet at one barrier per z step).  OPENB selects the open

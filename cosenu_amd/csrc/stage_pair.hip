@@ -1,0 +1,395 @@
+// FAST mode, two RK4 stages per sweep (variant 4): stages 1+2 in one launch,
+// stages 3+4 in the next.
+//
+// The split-lane team of the stage kernel (VP/32 wavefronts covering all
+// velocity bins of a z range, lanes 0-31 carrying ee, xx, ex_re, ex_im and
+// lanes 32-63 the barred fields of 32 bins) marches along z with two register
+// windows: the first stage's input, read from HBM, and the first stage's
+// output, produced in registers.  The second stage runs 4 cells behind the
+// first, so its window always holds first-stage values that are already
+// computed; the intermediate state never goes through HBM unless a later
+// stage needs it (Y2 for stage 4).  The first stage covers the team's cells
+// plus 4 on each side (recomputed by both neighbours), which the second stage
+// needs for its stencil.
+//
+// Bytes per cell and step: stages 1+2 read y (the window) and write Y2 and
+// Y3; stages 3+4 read Y3 (the window), y and Y2 and write the new state:
+// 64 + 128 + 192 + 64 = 448 B instead of 832 B for four separate stage
+// launches (plus the halo rows of the windows).
+//
+// The new state is written to a second buffer (the stages 3+4 launch reads y
+// in neighbouring teams' halos while it runs), and the state buffers swap
+// roles after every step.
+//
+// Moments: the first stage's input moments are reduced when a row enters its
+// window (as in the stage kernel); the second stage's input moments are
+// reduced from the first stage's output row as it is produced.  Each goes
+// through its own LDS ring of 8 rows and is read 4 iterations later; one
+// barrier per z step serves both.
+#include "fast_math.hpp"
+
+#include <cstdio>
+
+namespace cosenu {
+namespace {
+
+using namespace fastk;
+
+// A1: the first stage of the pair (1 or 3), A2: the second (2 or 4) of the
+// same run.  PAIR 0: stages 1+2, PAIR 1: stages 3+4.
+template <int NW, int SCHEME, int PAIR>
+__device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &A2, int L, int j0,
+                                          int jend, bool active, double (*ra)[NW][8],
+                                          double (*rb)[NW][8])
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wit = wave % NW;
+    const int h = lane >> 5;
+    const int v = wit * 32 + (lane & 31);
+    const int VP = A1.VP, nz = A1.nz;
+    const size_t zs = (size_t)8 * VP;
+    const double sig = h ? -1.0 : 1.0;
+    j0 = __builtin_amdgcn_readfirstlane(j0);
+    jend = __builtin_amdgcn_readfirstlane(jend);
+
+    const double vi = A1.vz[v];
+    const bool pos = vi > 0.0;
+    const bool anyp = __any(vi > 0.0), anyn = __any(vi < 0.0);
+    const int dir = anyn ? (anyp ? 0 : -1) : 1;
+    const bool live = v < A1.nvz;
+    const bool store = active && live;
+    const int loff = 4 * h * VP + v; // this lane's first field within a z row
+
+    auto zrow = [&](int m) -> int {
+        int w = m < 0 ? m + nz : m; // m >= -9 > -nz
+        while (w >= nz)
+            w -= nz;
+        return w;
+    };
+    auto load4 = [&](const double *base, int m, double o[4]) {
+        const double *r = base + (size_t)zrow(m) * zs + loff;
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            o[f] = r[(size_t)f * VP];
+    };
+
+    const int flags = A1.flags;
+    const bool coll = (flags & kFlagColl) != 0;
+    const bool vac = (flags & kFlagVac) != 0, mat = (flags & kFlagMat) != 0;
+    const bool advect = !(flags & kFlagAdvecOff);
+    const bool ko5 = (flags & kFlagKO5) != 0;
+    const double fac = -vi / A1.dz;
+    const double fdfac = -vi / (12.0 * A1.dz);
+    const double ko_eps = ko5 ? -A1.ko / A1.dz / 64.0 : -A1.ko / A1.dz / 16.0;
+    const double mudv = A1.mu * A1.dv;
+    const double st = A1.st * A1.pmo, ct = A1.ct * A1.pmo;
+
+    // partial moments of 4 field values of this lane into ring[row & 7]
+    auto moments = [&](double (*ring)[NW][8], int row, double p, double q, double r, double s) {
+        const double S = live ? s : 0.0;
+        const double R = live ? sig * r : 0.0;
+        const double D = live ? sig * (p - q) : 0.0;
+        const double m6[6] = {S, vi * S, R, vi * R, D, vi * D};
+        const double t = wave_sum6(m6, lane);
+        const int idx = (kMomentLane >> (4 * (lane >> 3))) & 15;
+        if ((lane & 7) == 0 && idx < 6)
+            ring[row & 7][wit][idx] = t;
+    };
+    // face value from a window, by the wavefront's upwind direction
+    auto face = [&](const double w[4][8], double F[4]) {
+        if (dir > 0) {
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                F[f] = flux_dir<1>(w[f], pos);
+        } else if (dir < 0) {
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                F[f] = flux_dir<-1>(w[f], pos);
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                F[f] = flux_dir<0>(w[f], pos);
+        }
+    };
+    // right-hand side at the window centre (position 3): z row `row` (wrapped
+    // into [0, nz)), moments in ring slot `rrow & 7` (rrow unwrapped, as
+    // when they were stored)
+    auto rhs = [&](const double w[4][8], const double F[4], const double Fp[4],
+                   double (*ring)[NW][8], int rrow, int row, double K[4]) {
+        const double p = w[0][3], qq = w[1][3], r = w[2][3], s = w[3][3];
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            K[f] = 0.0;
+        if (advect) {
+            if (SCHEME == 0) {
+#pragma unroll
+                for (int f = 0; f < 4; f++)
+                    K[f] = fac * (F[f] - Fp[f]);
+            } else {
+#pragma unroll
+                for (int f = 0; f < 4; f++)
+                    K[f] = fdfac * ((w[f][1] - w[f][5]) - 8.0 * (w[f][2] - w[f][4]));
+            }
+        }
+        if (vac) {
+            K[0] += -2.0 * st * s;
+            K[1] += 2.0 * st * s;
+            K[2] += -2.0 * ct * s;
+            K[3] += 2.0 * ct * r + st * (p - qq);
+        }
+        if (mat) {
+            const double hm = sig * A1.Hm[row];
+            K[2] += hm * s;
+            K[3] -= hm * r;
+        }
+        if (coll) {
+            double M[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int ww = 0; ww < NW; ww++) {
+                const double *slot = ring[rrow & 7][ww];
+#pragma unroll
+                for (int m = 0; m < 6; m++)
+                    M[m] += slot[m];
+            }
+            const double S = mudv * (M[0] - vi * M[1]);
+            const double R = mudv * (M[2] - vi * M[3]);
+            const double D = mudv * (M[4] - vi * M[5]);
+            const double I1 = 2.0 * (r * S - sig * s * R);
+            K[0] += I1;
+            K[1] -= I1;
+            K[2] += (qq - p) * S + sig * s * D;
+            K[3] += sig * ((p - qq) * R - r * D);
+        }
+        if (SCHEME == 1) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                const double *u = w[f];
+                if (ko5)
+                    K[f] += ko_eps * ((u[0] + u[6]) - 6.0 * (u[1] + u[5]) + 15.0 * (u[2] + u[4]) -
+                                      20.0 * u[3]);
+                else
+                    K[f] += ko_eps * ((u[1] + u[5]) - 4.0 * (u[2] + u[4]) + 6.0 * u[3]);
+            }
+        }
+    };
+
+    // First stage: cells [j0-4, jend+4); second stage: cells [j0, jend),
+    // 4 behind.  Iteration i: first stage at cell i, second at cell i-4;
+    // i = j0-5 computes the first stage's upstream face only.
+    const int ibeg = j0 - 5, iend = j0 + L + 4; // the same trip count in every team
+    double wa[4][8], wb[4][8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        double t[4];
+        load4(A1.yin, ibeg - 3 + k, t);
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            wa[f][k] = t[f];
+            wb[f][k] = 0.0;
+        }
+    }
+    if (coll) {
+        moments(ra, ibeg + 1, wa[0][4], wa[1][4], wa[2][4], wa[3][4]);
+        moments(ra, ibeg + 2, wa[0][5], wa[1][5], wa[2][5], wa[3][5]);
+        moments(ra, ibeg + 3, wa[0][6], wa[1][6], wa[2][6], wa[3][6]);
+    }
+    double Fa[4] = {0.0, 0.0, 0.0, 0.0}, Fb[4] = {0.0, 0.0, 0.0, 0.0};
+
+    for (int i = ibeg; i < iend; i++) {
+        double nxt[4];
+        load4(A1.yin, i + 5, nxt);
+        const int jb = i - 4;
+        const bool cell_a = i >= j0 - 4 && i < jend + 4;
+        const bool face_b = jb >= j0 - 1 && jb < jend;
+        const bool cell_b = jb >= j0 && jb < jend;
+        const int ra_row = zrow(i), rb_row = zrow(jb);
+        // pointwise operands: y at the first stage's cell (stage 3), and y,
+        // Y2, Y3 at the second stage's cell (y for stage 2; all at stage 4)
+        double ya[4], yb[4], y2b[4], y3b[4];
+        if (PAIR == 1 && cell_a)
+            load4(A1.y, i, ya);
+        if (cell_b) {
+            load4(A2.y, jb, yb);
+            if (PAIR == 1) {
+                load4(A2.y2, jb, y2b);
+                load4(A2.y3, jb, y3b);
+            }
+        }
+        if (coll)
+            moments(ra, i + 4, wa[0][7], wa[1][7], wa[2][7], wa[3][7]);
+        double F[4] = {0.0, 0.0, 0.0, 0.0};
+        if (SCHEME == 0 && advect)
+            face(wa, F);
+
+        __syncthreads();
+
+        // first stage at cell i: its output becomes the newest row of wb
+        double oa[4] = {0.0, 0.0, 0.0, 0.0};
+        if (cell_a) {
+            double K[4];
+            rhs(wa, F, Fa, ra, i, ra_row, K);
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                oa[f] = (PAIR == 0 ? wa[f][3] : ya[f]) + A1.a * K[f];
+            if (PAIR == 0 && store && i >= j0 && i < jend) { // Y2, read by stage 4
+                double *o = A1.yout + (size_t)ra_row * zs + loff;
+#pragma unroll
+                for (int f = 0; f < 4; f++)
+                    o[(size_t)f * VP] = oa[f];
+            }
+            if (coll)
+                moments(rb, i, oa[0], oa[1], oa[2], oa[3]);
+        }
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            Fa[f] = F[f];
+#pragma unroll
+            for (int k = 0; k < 7; k++)
+                wb[f][k] = wb[f][k + 1];
+            wb[f][7] = oa[f];
+        }
+        // second stage at cell jb: window wb holds rows jb-3 .. jb+4
+        if (face_b) {
+            double G[4] = {0.0, 0.0, 0.0, 0.0};
+            if (SCHEME == 0 && advect)
+                face(wb, G);
+            if (cell_b) {
+                double K[4];
+                rhs(wb, G, Fb, rb, jb, rb_row, K);
+                if (store) {
+                    double *o = A2.yout + (size_t)rb_row * zs + loff;
+#pragma unroll
+                    for (int f = 0; f < 4; f++) {
+                        double out;
+                        if (PAIR == 0) {
+                            out = yb[f] + A2.a * K[f];
+                        } else {
+                            const double y0 = yb[f];
+                            const double d =
+                                ((y2b[f] - y0) + 2.0 * (y3b[f] - y0)) + (wb[f][3] - y0);
+                            out = y0 + (d * (1.0 / 3.0) + A2.a * K[f]);
+                        }
+                        o[(size_t)f * VP] = out;
+                    }
+                }
+            }
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                Fb[f] = G[f];
+        }
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+#pragma unroll
+            for (int k = 0; k < 7; k++)
+                wa[f][k] = wa[f][k + 1];
+            wa[f][7] = nxt[f];
+        }
+    }
+}
+
+template <int NW, int SCHEME, int PAIR>
+__global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2)))
+void cosenu_pair_stage(StageArgs A1, StageArgs A2, int L, int nchunks, int nteams)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW; // teams per block
+    __shared__ double ring_a[TPB][8][NW][8];
+    __shared__ double ring_b[TPB][8][NW][8];
+    const int team = (threadIdx.x >> 6) / NW;
+    const int g = xcd_block_order() * TPB + team;
+    if (nteams > 0) {
+        const bool active = g < nteams;
+        const int j0 = active ? (int)(((long)g * A1.nz) / nteams) : 0;
+        const int j1 = active ? (int)(((long)(g + 1) * A1.nz) / nteams) : 0;
+        pair_body<NW, SCHEME, PAIR>(A1, A2, L, j0, j1, active, ring_a[team], ring_b[team]);
+    } else {
+        const bool active = g < nchunks;
+        const int j0 = (active ? g : 0) * L;
+        pair_body<NW, SCHEME, PAIR>(A1, A2, L, j0, j0 + L, active, ring_a[team], ring_b[team]);
+    }
+}
+
+template <int NW, int SCHEME>
+hipError_t launch_nw(const StageArgs &a1, const StageArgs &a2, int pair, int L, int nteams,
+                     hipStream_t s)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
+    int nchunks = 0, work;
+    if (nteams > 0) {
+        L = (a1.nz + nteams - 1) / nteams;
+        work = nteams;
+    } else {
+        nchunks = a1.nz / L;
+        work = nchunks;
+    }
+    dim3 grid((work + TPB - 1) / TPB), block(64 * NW * TPB);
+    if (pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 0>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+    else
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 1>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+    return hipGetLastError();
+}
+
+template <int SCHEME>
+hipError_t launch_scheme(const StageArgs &a1, const StageArgs &a2, int pair, int L, int nteams,
+                         hipStream_t s)
+{
+    switch (a1.VP) {
+    case 64: return launch_nw<2, SCHEME>(a1, a2, pair, L, nteams, s);
+    case 128: return launch_nw<4, SCHEME>(a1, a2, pair, L, nteams, s);
+    case 256: return launch_nw<8, SCHEME>(a1, a2, pair, L, nteams, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int NW, int SCHEME>
+int resident_teams_nw()
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
+    int nb = 0, dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&cosenu_pair_stage<NW, SCHEME, 1>),
+            64 * NW * TPB, 0) != hipSuccess)
+        return 0;
+    return nb * prop.multiProcessorCount * TPB;
+}
+
+} // namespace
+
+int pair_supported(int VP, int bc) { return (VP == 64 || VP == 128 || VP == 256) && bc == 0; }
+
+hipError_t launch_stage_pair(const StageArgs &a1, const StageArgs &a2, int pair, int L,
+                             int nteams, hipStream_t s)
+{
+    if (!pair_supported(a1.VP, a1.bc) || a1.nz < 16 || (pair != 0 && pair != 1))
+        return hipErrorInvalidValue;
+    if (nteams > 0) {
+        if (nteams > a1.nz)
+            return hipErrorInvalidValue;
+    } else if (L < 1 || a1.nz % L != 0) {
+        return hipErrorInvalidValue;
+    }
+    return a1.scheme == 0 ? launch_scheme<0>(a1, a2, pair, L, nteams, s)
+                          : launch_scheme<1>(a1, a2, pair, L, nteams, s);
+}
+
+int pair_resident_teams(int VP, int scheme)
+{
+    switch (VP) {
+    case 64: return scheme == 0 ? resident_teams_nw<2, 0>() : resident_teams_nw<2, 1>();
+    case 128: return scheme == 0 ? resident_teams_nw<4, 0>() : resident_teams_nw<4, 1>();
+    case 256: return scheme == 0 ? resident_teams_nw<8, 0>() : resident_teams_nw<8, 1>();
+    default: return 0;
+    }
+}
+
+const char *stage_pair_kernel_name(int pair)
+{
+    static char buf[64];
+    snprintf(buf, sizeof(buf), "cosenu_pair_stage<*, *, %d>", pair);
+    return buf;
+}
+
+} // namespace cosenu

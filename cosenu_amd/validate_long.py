@@ -14,6 +14,7 @@ from t = 0.  Prints one JSON line at the end.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import sys
 import time
@@ -24,18 +25,31 @@ from . import FAST, FD, FV, STATE, STATE0, STRICT, Simulation
 from .initial import workload
 
 
+def subsampled(wl, nz: int):
+    """The workload on a periodic domain of nz cells with the same dz, centred
+    on z = 0 (physics, initial-state recipe and seed unchanged)."""
+    p = wl.params
+    half = 0.5 * nz * (p.z1 - p.z0) / p.nz
+    wl.params = dataclasses.replace(p, nz=nz, z0=-half, z1=half)
+    return wl
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--scheme", choices=["fv", "fd"], default="fv")
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--every", type=int, default=2000)
+    ap.add_argument("--nz", type=int, default=0,
+                    help="z-subsampled periodic domain: this many cells at the workload's dz")
     args = ap.parse_args(argv)
     scheme = FV if args.scheme == "fv" else FD
 
     sims = {}
     for name, mode in (("strict", STRICT), ("fast", FAST)):
         wl = workload(args.workload, mode=mode, scheme=scheme)
+        if args.nz:
+            wl = subsampled(wl, args.nz)
         a, b, G1, bG1 = wl.factors()
         sim = Simulation(wl.params)
         sim.set_profile_v(G1, bG1)
@@ -76,6 +90,7 @@ def main(argv=None):
         print(json.dumps({"progress": rec}), flush=True)
     last = history[-1]
     out = {"workload": args.workload, "scheme": args.scheme, "nz": p.nz, "nvz": p.nvz,
+           "z": [p.z0, p.z1],
            "steps": args.steps, "pass_1e-10": bool(last["pee_linf"] <= 1e-10 and last["pee_l2"] <= 1e-10
                                                    and last["trace_rel_diff"] <= 1e-10
                                                    and last["lepton_rel_diff"] <= 1e-10),

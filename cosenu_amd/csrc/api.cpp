@@ -546,8 +546,9 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             c->variant = tv.variant;
             chunk = tv.chunk;
         } else {
-            // split-lane kernel over contiguous z ranges, one team per resident slot
-            c->variant = 2;
+            // two stages per launch where it applies, else the split-lane
+            // kernel; contiguous z ranges, one team per resident slot
+            c->variant = pair_supported(c->VP, p->bc) && c->nz >= 16 ? 4 : 2;
             chunk = -1;
         }
         int ev = env_int("COSENU_FAST_VARIANT", 0), ez = env_int("COSENU_ZCHUNK", 0);

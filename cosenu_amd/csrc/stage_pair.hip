@@ -15,7 +15,10 @@
 // Bytes per cell and step: stages 1+2 read y (the window) and write Y2 and
 // Y3; stages 3+4 read Y3 (the window), y and Y2 and write the new state:
 // 64 + 128 + 192 + 64 = 448 B instead of 832 B for four separate stage
-// launches (plus the halo rows of the windows).
+// launches (plus the halo rows of the windows).  The second stage's pointwise
+// use of the first stage's input (y at stage 2, Y3 at stage 4) comes from the
+// row that has just left the window, kept in registers.  Stage 4 reads y a
+// second time, 4 rows after stage 3 did.
 //
 // The new state is written to a second buffer (the stages 3+4 launch reads y
 // in neighbouring teams' halos while it runs), and the state buffers swap
@@ -24,8 +27,12 @@
 // Moments: the first stage's input moments are reduced when a row enters its
 // window (as in the stage kernel); the second stage's input moments are
 // reduced from the first stage's output row as it is produced.  Each goes
-// through its own LDS ring of 8 rows and is read 4 iterations later; one
-// barrier per z step serves both.
+// through its own LDS ring of kRing rows and is read 4 iterations later.  A
+// barrier every kSync iterations suffices: between the store of a row's
+// moments and their use 4 iterations later lies at least one barrier, and so
+// does between the last use of a ring slot and its next store (kRing - 5 >=
+// kSync iterations apart), so the waves of a team drift up to kSync - 1
+// iterations apart instead of meeting at every z step.
 #include "fast_math.hpp"
 
 #include <cstdio>
@@ -35,12 +42,16 @@ namespace {
 
 using namespace fastk;
 
+constexpr int kRing = 16, kSync = 4;
+static_assert(kRing - 5 >= kSync && kSync <= 4, "moment ring too shallow for the barrier spacing");
+
 // A1: the first stage of the pair (1 or 3), A2: the second (2 or 4) of the
 // same run.  PAIR 0: stages 1+2, PAIR 1: stages 3+4.
 template <int NW, int SCHEME, int PAIR>
 __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &A2, int L, int j0,
                                           int jend, bool active, double (*ra)[NW][8],
                                           double (*rb)[NW][8])
+// ra, rb: [kRing][NW][8]
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wit = wave % NW;
@@ -84,7 +95,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     const double mudv = A1.mu * A1.dv;
     const double st = A1.st * A1.pmo, ct = A1.ct * A1.pmo;
 
-    // partial moments of 4 field values of this lane into ring[row & 7]
+    // partial moments of 4 field values of this lane into ring[row % kRing]
     auto moments = [&](double (*ring)[NW][8], int row, double p, double q, double r, double s) {
         const double S = live ? s : 0.0;
         const double R = live ? sig * r : 0.0;
@@ -93,7 +104,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         const double t = wave_sum6(m6, lane);
         const int idx = (kMomentLane >> (4 * (lane >> 3))) & 15;
         if ((lane & 7) == 0 && idx < 6)
-            ring[row & 7][wit][idx] = t;
+            ring[row & (kRing - 1)][wit][idx] = t;
     };
     // face value from a window, by the wavefront's upwind direction
     auto face = [&](const double w[4][8], double F[4]) {
@@ -112,7 +123,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         }
     };
     // right-hand side at the window centre (position 3): z row `row` (wrapped
-    // into [0, nz)), moments in ring slot `rrow & 7` (rrow unwrapped, as
+    // into [0, nz)), moments in ring slot `rrow % kRing` (rrow unwrapped, as
     // when they were stored)
     auto rhs = [&](const double w[4][8], const double F[4], const double Fp[4],
                    double (*ring)[NW][8], int rrow, int row, double K[4]) {
@@ -146,7 +157,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             double M[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int ww = 0; ww < NW; ww++) {
-                const double *slot = ring[rrow & 7][ww];
+                const double *slot = ring[rrow & (kRing - 1)][ww];
 #pragma unroll
                 for (int m = 0; m < 6; m++)
                     M[m] += slot[m];
@@ -194,6 +205,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         moments(ra, ibeg + 3, wa[0][6], wa[1][6], wa[2][6], wa[3][6]);
     }
     double Fa[4] = {0.0, 0.0, 0.0, 0.0}, Fb[4] = {0.0, 0.0, 0.0, 0.0};
+    double wprev[4] = {0.0, 0.0, 0.0, 0.0}; // the row before the window
 
     for (int i = ibeg; i < iend; i++) {
         double nxt[4];
@@ -204,16 +216,15 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         const bool cell_b = jb >= j0 && jb < jend;
         const int ra_row = zrow(i), rb_row = zrow(jb);
         // pointwise operands: y at the first stage's cell (stage 3), and y,
-        // Y2, Y3 at the second stage's cell (y for stage 2; all at stage 4)
-        double ya[4], yb[4], y2b[4], y3b[4];
+        // Y2, Y3 at the second stage's cell (y for stage 2; all at stage 4).
+        // Row jb of the first stage's input (y for stages 1+2, Y3 for 3+4)
+        // left the window one iteration ago and is kept in `wprev`.
+        double ya[4], yb[4], y2b[4];
         if (PAIR == 1 && cell_a)
             load4(A1.y, i, ya);
-        if (cell_b) {
+        if (cell_b && PAIR == 1) {
             load4(A2.y, jb, yb);
-            if (PAIR == 1) {
-                load4(A2.y2, jb, y2b);
-                load4(A2.y3, jb, y3b);
-            }
+            load4(A2.y2, jb, y2b);
         }
         if (coll)
             moments(ra, i + 4, wa[0][7], wa[1][7], wa[2][7], wa[3][7]);
@@ -221,7 +232,8 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         if (SCHEME == 0 && advect)
             face(wa, F);
 
-        __syncthreads();
+        if (((i - ibeg) & (kSync - 1)) == 0) // the same iterations in every team
+            __syncthreads();
 
         // first stage at cell i: its output becomes the newest row of wb
         double oa[4] = {0.0, 0.0, 0.0, 0.0};
@@ -262,11 +274,11 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
                     for (int f = 0; f < 4; f++) {
                         double out;
                         if (PAIR == 0) {
-                            out = yb[f] + A2.a * K[f];
+                            out = wprev[f] + A2.a * K[f];
                         } else {
                             const double y0 = yb[f];
                             const double d =
-                                ((y2b[f] - y0) + 2.0 * (y3b[f] - y0)) + (wb[f][3] - y0);
+                                ((y2b[f] - y0) + 2.0 * (wprev[f] - y0)) + (wb[f][3] - y0);
                             out = y0 + (d * (1.0 / 3.0) + A2.a * K[f]);
                         }
                         o[(size_t)f * VP] = out;
@@ -279,6 +291,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         }
 #pragma unroll
         for (int f = 0; f < 4; f++) {
+            wprev[f] = wa[f][0];
 #pragma unroll
             for (int k = 0; k < 7; k++)
                 wa[f][k] = wa[f][k + 1];
@@ -292,8 +305,8 @@ __global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_pe
 void cosenu_pair_stage(StageArgs A1, StageArgs A2, int L, int nchunks, int nteams)
 {
     constexpr int TPB = NW >= 4 ? 1 : 4 / NW; // teams per block
-    __shared__ double ring_a[TPB][8][NW][8];
-    __shared__ double ring_b[TPB][8][NW][8];
+    __shared__ double ring_a[TPB][kRing][NW][8];
+    __shared__ double ring_b[TPB][kRing][NW][8];
     const int team = (threadIdx.x >> 6) / NW;
     const int g = xcd_block_order() * TPB + team;
     if (nteams > 0) {

@@ -172,9 +172,13 @@ int cosenu_snapshot_wait(cosenu_ctx *ctx);
 void *cosenu_host_alloc(size_t bytes);
 void cosenu_host_free(void *p);
 
-/* Per-kernel timing with HIP events recorded around every stage launch on
- * the context's stream.  ms[s] = total milliseconds of RK stage s (0..3)
- * since timing was enabled, counts[s] = launches timed.  Enabling clears. */
+/* Per-kernel timing with HIP events recorded around every launch of a step
+ * on the context's stream.  ms[k] = total milliseconds of launch k of the
+ * step since timing was enabled, counts[k] = launches timed: launch k = RK
+ * stage k+1 for the one-stage kernels; with two stages per launch (FAST
+ * variant 4) launch 0 = stages 1+2, launch 1 = stages 3+4 and slots 2, 3
+ * stay empty.  Enabling clears.  cosenu_stage_kernel_name(ctx, k) names the
+ * kernel of launch k. */
 int cosenu_set_timing(cosenu_ctx *ctx, int enable);
 int cosenu_get_timing(cosenu_ctx *ctx, double ms[4], long long counts[4]);
 

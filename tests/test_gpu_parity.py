@@ -228,7 +228,11 @@ def test_snapshot_pee_async(gpu):
     assert np.array_equal(snap, s3[0] / s0[0])
 
 
-def test_timing_reports_every_stage(gpu):
+@pytest.mark.parametrize("variant", ["2", "4"])
+def test_timing_reports_every_launch(gpu, monkeypatch, variant):
+    """One timing slot per launch of a step: four one-stage launches, or two
+    two-stage launches."""
+    monkeypatch.setenv("COSENU_FAST_VARIANT", variant)
     po, pc = _pair(O.FV, O.PERIODIC, O.VAC | O.COLL, 512, 128, C.FAST)
     _, _, s = _state(po, 20)
     with C.Simulation(pc) as sim:
@@ -236,7 +240,10 @@ def test_timing_reports_every_stage(gpu):
         sim.set_timing(True)
         sim.step(5)
         ms, cnt = sim.timing()
-    assert cnt == [5, 5, 5, 5] and all(m > 0 for m in ms)
+        names = [sim.stage_kernel_name(k) for k in range(4)]
+    n = 4 if variant == "2" else 2
+    assert cnt == [5] * n + [0] * (4 - n)
+    assert all(m > 0 for m in ms[:n]) and all(names[:n]) and not any(names[n:])
 
 
 def test_errors_are_reported_not_fatal(gpu):

@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FP64 z·v cell-updates/s + HBM roofline %; surrogate vs solver P_ee sweep speedup"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_GINST = 1024 * 2.4 / 4  # wave64 VALU instructions per ns at the peak clock
 # Bytes per cell of one RK4 step under the classic accumulator form (read
 # the stage input, y and the accumulator, write the accumulator and the next
 # stage input, 8 FP64 fields each): the figure the roofline is defined with.
@@ -160,6 +161,20 @@ def main():
         if rec.get("config") == launch:
             traffic = rec.get(dom)
 
+    # VALU issue rate of the dominant kernel: wavefront VALU instructions per
+    # launch from a `rocprofv3 --pmc SQ_INSTS_VALU` run recorded under
+    # profiles/ (same launch configuration), over the measured duration,
+    # against 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 FP64 instruction.
+    valu = None
+    vfile = os.path.join(ROOT, "profiles", f"valu_{args.workload}_{args.scheme}.json")
+    if os.path.exists(vfile):
+        with open(vfile) as f:
+            rec = json.load(f)
+        if rec.get("config") == launch and dom in rec:
+            ginst = rec[dom] / (dom_ms * 1e-3) / 1e9
+            valu = {"wave_insts_per_launch": rec[dom], "achieved": ginst, "peak": VALU_PEAK_GINST,
+                    "unit": "G wave-instructions/s", "frac": ginst / VALU_PEAK_GINST}
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -187,7 +202,8 @@ def main():
                      "stage_avg_ms": avg,
                      "step_kernels_GBs": step_gbs,
                      "step_kernels_frac": step_gbs / HBM_PEAK_GBS,
-                     "frac_at_1088B_per_cell_update": value * BYTES_PER_CELL_STEP_DEF / 1e9 / HBM_PEAK_GBS},
+                     "frac_at_1088B_per_cell_update": value * BYTES_PER_CELL_STEP_DEF / 1e9 / HBM_PEAK_GBS,
+                     "valu": valu},
     }
     sim.close()
     if not args.no_cpu:

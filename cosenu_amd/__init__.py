@@ -53,7 +53,8 @@ class _Info(ctypes.Structure):
 class _BatchInfo(ctypes.Structure):
     _fields_ = [("nruns", ctypes.c_int), ("ngroups", ctypes.c_int), ("cells", ctypes.c_longlong),
                 ("steps_done", ctypes.c_longlong), ("device_bytes", ctypes.c_size_t),
-                ("work_items", ctypes.c_longlong), ("z_chunk", ctypes.c_int)]
+                ("work_items", ctypes.c_longlong), ("z_chunk", ctypes.c_int),
+                ("fast_variant", ctypes.c_int)]
 
 
 class _Diag(ctypes.Structure):

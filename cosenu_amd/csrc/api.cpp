@@ -338,6 +338,7 @@ struct cosenu_ctx {
     int z_chunk = 0, variant = 2;
     int nteams = 0; // > 0: contiguous-range mode with this many teams
     bool stream_ok = false; // the barrier-free kernel (variant 3) applies
+    bool uniform = false;   // nvz == VP, every group of 32 bins of one sign of v
     double *mom[4] = {nullptr, nullptr, nullptr, nullptr}; // its moments of y, A, B, acc
     View view;
     bool timing = false;
@@ -441,7 +442,7 @@ hipError_t launch_pair(const cosenu_ctx *c, int pair, hipStream_t s)
     StageArgs a1 = stage_args(c, pair == 0 ? 1 : 3), a2 = stage_args(c, pair == 0 ? 2 : 4);
     if (pair == 1)
         a2.yout = c->acc;
-    return launch_stage_pair(a1, a2, pair, c->z_chunk, c->nteams, s);
+    return launch_stage_pair(a1, a2, pair, c->z_chunk, c->nteams, c->uniform, s);
 }
 
 void swap_state(cosenu_ctx *c)
@@ -536,7 +537,8 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
     const size_t n_state = (size_t)c->nz * 8 * c->VP;
     std::vector<double> vz, Hm;
     fill_grids(*p, c->d, c->VP, vz, Hm);
-    c->stream_ok = p->mode == COSENU_FAST && stream_supported(c->nvz, c->VP, p->bc, vz.data());
+    c->uniform = stream_supported(c->nvz, c->VP, COSENU_PERIODIC, vz.data()) != 0;
+    c->stream_ok = p->mode == COSENU_FAST && c->uniform && p->bc == COSENU_PERIODIC;
     if (p->mode == COSENU_FAST) {
         // z_chunk < 0 in the tune table = contiguous-range mode with that many
         // times the resident teams
@@ -1041,8 +1043,10 @@ struct Group {
     double *y = nullptr, *acc = nullptr, *A = nullptr, *B = nullptr, *y0 = nullptr;
     double *prof = nullptr, *vz = nullptr, *Hm = nullptr;
     BatchRun *d_runs = nullptr;
+    BatchRun *d_runs_sw = nullptr; // the same with y and acc swapped (two-stage launches)
     int2 *d_items = nullptr;
     int nitems = 0;
+    bool uniform = false; // every run: nvz == VP, groups of 32 bins of one sign of v
 };
 
 } // namespace
@@ -1064,6 +1068,11 @@ struct cosenu_batch {
     long long cells = 0, steps = 0;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    // two stages per launch: the state of run r is in buffer `parity` of
+    // (run_y[r], run_sp[r]) and the two alternate every step
+    bool pairs = false;
+    int parity = 0, graph_parity = 0;
+    std::vector<double *> run_y, run_sp;
 };
 
 namespace {
@@ -1077,6 +1086,8 @@ void release_batch(cosenu_batch *b)
                 (void)hipFree(p);
         if (g.d_runs)
             (void)hipFree(g.d_runs);
+        if (g.d_runs_sw)
+            (void)hipFree(g.d_runs_sw);
         if (g.d_items)
             (void)hipFree(g.d_items);
     }
@@ -1092,6 +1103,16 @@ void release_batch(cosenu_batch *b)
 
 int batch_enqueue_step(cosenu_batch *b, hipStream_t s)
 {
+    if (b->pairs) {
+        for (int pair = 0; pair < 2; pair++)
+            for (Group &g : b->groups)
+                HIP_TRY(launch_stage_pair_batch(b->parity ? g.d_runs_sw : g.d_runs, g.d_items,
+                                                g.nitems, g.VP, g.scheme, g.L, pair, g.uniform, s));
+        b->parity ^= 1;
+        for (int r = 0; r < b->nruns; r++)
+            b->views[r].y = b->parity ? b->run_sp[r] : b->run_y[r];
+        return 0;
+    }
     for (int st = 1; st <= 4; st++)
         for (Group &g : b->groups)
             HIP_TRY(launch_stage_fast_batch(g.d_runs, g.d_items, g.nitems, g.VP, g.scheme, g.L, st, s));
@@ -1109,13 +1130,20 @@ int group_chunk(const cosenu_batch *b, const Group &g)
     if (want <= 0 && tune_lookup(g.VP, g.scheme, -1, &tv))
         want = tv.chunk;
     if (want <= 0) {
-        // about two work items per resident team, chunks of 8 to 32 cells:
-        // a few runs still spread over the whole device
         long rows = 0;
         for (int r : g.runs)
             rows += b->P[r].nz;
-        const long target = 2L * std::max(1, fast_resident_teams(g.VP, g.scheme, 0));
-        want = (int)std::max(8L, std::min(32L, rows / target));
+        if (b->pairs) {
+            // one work item per resident team, 32 to 512 cells (each item
+            // recomputes 8 rows of halo for its second stage)
+            const long target = std::max(1, pair_resident_teams(g.VP, g.scheme));
+            want = (int)std::max(32L, std::min(512L, rows / target));
+        } else {
+            // about two work items per resident team, chunks of 8 to 32
+            // cells: a few runs still spread over the whole device
+            const long target = 2L * std::max(1, fast_resident_teams(g.VP, g.scheme, 0));
+            want = (int)std::max(8L, std::min(32L, rows / target));
+        }
     }
     int L = std::min(want, gcd);
     while (L > 1 && gcd % L)
@@ -1153,6 +1181,8 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
     b->off_state.resize(nruns);
     b->off_prof.resize(nruns);
     b->views.resize(nruns);
+    b->run_y.assign(nruns, nullptr);
+    b->run_sp.assign(nruns, nullptr);
     b->have_profile.assign(nruns, false);
     b->have_state0.assign(nruns, false);
     // group by (VP, scheme)
@@ -1181,6 +1211,12 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
         g.n_hm += runs[r].nz;
         b->cells += (long long)runs[r].nz * runs[r].nvz;
     }
+    // two stages per launch unless COSENU_BATCH_VARIANT=2 (one stage per launch)
+    b->pairs = env_int("COSENU_BATCH_VARIANT", 4) != 2;
+    for (int r = 0; r < nruns; r++)
+        b->pairs = b->pairs && runs[r].nz >= 16;
+    for (const Group &g : b->groups)
+        b->pairs = b->pairs && pair_supported(g.VP, COSENU_PERIODIC);
     int rc = 0;
     for (Group &g : b->groups) {
         if (!rc) rc = dev_alloc(b->bytes, &g.y, g.n_state);
@@ -1212,6 +1248,7 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
             e = hipMemset(g.prof, 0, g.n_prof * sizeof(double));
         // per-run tables
         g.L = group_chunk(b, g);
+        g.uniform = true;
         std::vector<BatchRun> hr;
         std::vector<double> vz_all, hm_all;
         std::vector<int2> items;
@@ -1222,6 +1259,7 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
             const Derived &d = b->D[r];
             std::vector<double> vz, Hm;
             fill_grids(p, d, g.VP, vz, Hm);
+            g.uniform = g.uniform && stream_supported(p.nvz, g.VP, COSENU_PERIODIC, vz.data());
             vz_all.insert(vz_all.end(), vz.begin(), vz.end());
             hm_all.insert(hm_all.end(), Hm.begin(), Hm.end());
             BatchRun br;
@@ -1236,6 +1274,8 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
             hr.push_back(br);
             for (int j0 = 0; j0 < p.nz; j0 += g.L)
                 items.push_back(make_int2((int)k, j0));
+            b->run_y[r] = br.y;
+            b->run_sp[r] = br.acc;
             View &v = b->views[r];
             v.y = br.y; v.y0 = g.y0 + so; v.prof = g.prof + b->off_prof[r];
             v.sa = br.A; v.sb = br.B;
@@ -1245,7 +1285,13 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
             ho += p.nz;
         }
         g.nitems = (int)items.size();
+        std::vector<BatchRun> hs = hr;
+        for (BatchRun &br : hs)
+            std::swap(br.y, br.acc);
         if (e == hipSuccess) e = hipMalloc((void **)&g.d_runs, hr.size() * sizeof(BatchRun));
+        if (e == hipSuccess) e = hipMalloc((void **)&g.d_runs_sw, hs.size() * sizeof(BatchRun));
+        if (e == hipSuccess)
+            e = hipMemcpy(g.d_runs_sw, hs.data(), hs.size() * sizeof(BatchRun), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc((void **)&g.d_items, items.size() * sizeof(int2));
         if (e == hipSuccess)
             e = hipMemcpy(g.d_runs, hr.data(), hr.size() * sizeof(BatchRun), hipMemcpyHostToDevice);
@@ -1321,14 +1367,17 @@ int cosenu_batch_step(cosenu_batch *b, int nsteps)
     if (!b || nsteps < 0)
         return fail(COSENU_EINVAL, "cosenu_batch_step: bad argument");
     HIP_TRY(hipSetDevice(b->device));
-    const int gsteps = env_int("COSENU_GRAPH_STEPS", 16);
+    int gsteps = env_int("COSENU_GRAPH_STEPS", 16);
+    if (b->pairs)
+        gsteps &= ~1; // an even number of steps ends on the buffers it started from
     int done = 0;
     if (gsteps > 0 && nsteps >= gsteps) {
-        if (!b->graph || b->graph_steps != gsteps) {
+        if (!b->graph || b->graph_steps != gsteps || b->graph_parity != b->parity) {
             if (b->graph) {
                 (void)hipGraphExecDestroy(b->graph);
                 b->graph = nullptr;
             }
+            b->graph_parity = b->parity;
             auto one = [b]() -> int { return batch_enqueue_step(b, b->stream); };
             RC_TRY(capture_graph(b->stream, gsteps, one, &b->graph));
             b->graph_steps = gsteps;
@@ -1388,6 +1437,7 @@ int cosenu_batch_get_info(cosenu_batch *b, cosenu_batch_info *info)
     for (const Group &g : b->groups)
         info->work_items += g.nitems;
     info->z_chunk = b->groups.empty() ? 0 : b->groups[0].L;
+    info->fast_variant = b->pairs ? 4 : 2;
     return 0;
 }
 

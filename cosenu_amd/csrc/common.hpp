@@ -140,9 +140,16 @@ const char *stage_stream_kernel_name(int stage);
 // alias a1.y (the new state goes to a second buffer).  Periodic boundaries,
 // VP in {64, 128, 256}.
 int pair_supported(int VP, int bc);
+// uniform: nvz == VP and every group of 32 bins has one sign of v
+// (stream_supported's test) -- selects kernels without padding masks.
 hipError_t launch_stage_pair(const StageArgs &a1, const StageArgs &a2, int pair, int z_chunk,
-                             int nteams, hipStream_t s);
+                             int nteams, bool uniform, hipStream_t s);
 int pair_resident_teams(int VP, int scheme);
+// Batched two-stage launch over work items (run, first z cell) of runs with
+// the same VP and scheme; each run's .y is its current state and .acc the
+// buffer that receives the new one (pair 1).
+hipError_t launch_stage_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
+                                   int scheme, int L, int pair, bool uniform, hipStream_t s);
 const char *stage_pair_kernel_name(int pair);
 const char *stage_fast_kernel_name(int VP, int stage, int variant);
 // Batched FAST stage over `nitems` work items (run index, first z cell) of

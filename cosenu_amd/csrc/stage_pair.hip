@@ -46,8 +46,10 @@ constexpr int kRing = 16, kSync = 4;
 static_assert(kRing - 5 >= kSync && kSync <= 4, "moment ring too shallow for the barrier spacing");
 
 // A1: the first stage of the pair (1 or 3), A2: the second (2 or 4) of the
-// same run.  PAIR 0: stages 1+2, PAIR 1: stages 3+4.
-template <int NW, int SCHEME, int PAIR>
+// same run.  PAIR 0: stages 1+2, PAIR 1: stages 3+4.  UNI: no padding bins
+// and every wavefront's bins of one sign of v (the common case): no masks
+// and no per-lane stencil selection.
+template <int NW, int SCHEME, int PAIR, bool UNI>
 __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &A2, int L, int j0,
                                           int jend, bool active, double (*ra)[NW][8],
                                           double (*rb)[NW][8])
@@ -66,8 +68,8 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     const double vi = A1.vz[v];
     const bool pos = vi > 0.0;
     const bool anyp = __any(vi > 0.0), anyn = __any(vi < 0.0);
-    const int dir = anyn ? (anyp ? 0 : -1) : 1;
-    const bool live = v < A1.nvz;
+    const int dir = UNI ? (anyp ? 1 : -1) : (anyn ? (anyp ? 0 : -1) : 1);
+    const bool live = UNI || v < A1.nvz;
     const bool store = active && live;
     const int loff = 4 * h * VP + v; // this lane's first field within a z row
 
@@ -112,7 +114,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
 #pragma unroll
             for (int f = 0; f < 4; f++)
                 F[f] = flux_dir<1>(w[f], pos);
-        } else if (dir < 0) {
+        } else if (UNI || dir < 0) {
 #pragma unroll
             for (int f = 0; f < 4; f++)
                 F[f] = flux_dir<-1>(w[f], pos);
@@ -300,7 +302,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     }
 }
 
-template <int NW, int SCHEME, int PAIR>
+template <int NW, int SCHEME, int PAIR, bool UNI>
 __global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_pair_stage(StageArgs A1, StageArgs A2, int L, int nchunks, int nteams)
 {
@@ -313,17 +315,59 @@ void cosenu_pair_stage(StageArgs A1, StageArgs A2, int L, int nchunks, int nteam
         const bool active = g < nteams;
         const int j0 = active ? (int)(((long)g * A1.nz) / nteams) : 0;
         const int j1 = active ? (int)(((long)(g + 1) * A1.nz) / nteams) : 0;
-        pair_body<NW, SCHEME, PAIR>(A1, A2, L, j0, j1, active, ring_a[team], ring_b[team]);
+        pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, j0, j1, active, ring_a[team], ring_b[team]);
     } else {
         const bool active = g < nchunks;
         const int j0 = (active ? g : 0) * L;
-        pair_body<NW, SCHEME, PAIR>(A1, A2, L, j0, j0 + L, active, ring_a[team], ring_b[team]);
+        pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, j0, j0 + L, active, ring_a[team], ring_b[team]);
     }
+}
+
+// Batched launch: work item k = (run, first z cell) of runs with the same VP
+// and scheme, all with chunk length L.  `runs` holds each run's current state
+// in .y and the spare in .acc (the host passes the array with the two
+// swapped after every step).
+template <int NW, int SCHEME, int PAIR, bool UNI>
+__global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2)))
+void cosenu_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int L)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
+    __shared__ double ring_a[TPB][kRing][NW][8];
+    __shared__ double ring_b[TPB][kRing][NW][8];
+    const int team = (threadIdx.x >> 6) / NW;
+    const int item = xcd_block_order() * TPB + team;
+    const bool active = item < nitems;
+    const int2 it = items[active ? item : 0];
+    const BatchRun &r = runs[it.x];
+    const StageArgs A1 = batch_stage_args(r, PAIR == 0 ? 1 : 3, true);
+    StageArgs A2 = batch_stage_args(r, PAIR == 0 ? 2 : 4, true);
+    if (PAIR == 1)
+        A2.yout = r.acc;
+    pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, it.y, it.y + L, active, ring_a[team], ring_b[team]);
+}
+
+template <int NW, int SCHEME>
+hipError_t launch_batch_nw(const BatchRun *runs, const int2 *items, int nitems, int L, int pair,
+                           bool uni, hipStream_t s)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
+    if (nitems <= 0)
+        return hipSuccess;
+    dim3 grid((nitems + TPB - 1) / TPB), block(64 * NW * TPB);
+    if (uni && pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_batch<NW, SCHEME, 0, true>), grid, block, 0, s, runs, items, nitems, L);
+    else if (uni)
+        hipLaunchKernelGGL((cosenu_pair_batch<NW, SCHEME, 1, true>), grid, block, 0, s, runs, items, nitems, L);
+    else if (pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_batch<NW, SCHEME, 0, false>), grid, block, 0, s, runs, items, nitems, L);
+    else
+        hipLaunchKernelGGL((cosenu_pair_batch<NW, SCHEME, 1, false>), grid, block, 0, s, runs, items, nitems, L);
+    return hipGetLastError();
 }
 
 template <int NW, int SCHEME>
 hipError_t launch_nw(const StageArgs &a1, const StageArgs &a2, int pair, int L, int nteams,
-                     hipStream_t s)
+                     bool uni, hipStream_t s)
 {
     constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
     int nchunks = 0, work;
@@ -335,21 +379,27 @@ hipError_t launch_nw(const StageArgs &a1, const StageArgs &a2, int pair, int L, 
         work = nchunks;
     }
     dim3 grid((work + TPB - 1) / TPB), block(64 * NW * TPB);
-    if (pair == 0)
-        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 0>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+    // uniform: no padding bins and each group of 32 bins of one sign of v
+    // (v0 = -v1 and nvz a multiple of 64 give that; `uniform` is the host's check)
+    if (uni && pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 0, true>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+    else if (uni)
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 1, true>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+    else if (pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 0, false>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
     else
-        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 1>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 1, false>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
     return hipGetLastError();
 }
 
 template <int SCHEME>
 hipError_t launch_scheme(const StageArgs &a1, const StageArgs &a2, int pair, int L, int nteams,
-                         hipStream_t s)
+                         bool uni, hipStream_t s)
 {
     switch (a1.VP) {
-    case 64: return launch_nw<2, SCHEME>(a1, a2, pair, L, nteams, s);
-    case 128: return launch_nw<4, SCHEME>(a1, a2, pair, L, nteams, s);
-    case 256: return launch_nw<8, SCHEME>(a1, a2, pair, L, nteams, s);
+    case 64: return launch_nw<2, SCHEME>(a1, a2, pair, L, nteams, uni, s);
+    case 128: return launch_nw<4, SCHEME>(a1, a2, pair, L, nteams, uni, s);
+    case 256: return launch_nw<8, SCHEME>(a1, a2, pair, L, nteams, uni, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -363,7 +413,7 @@ int resident_teams_nw()
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void *>(&cosenu_pair_stage<NW, SCHEME, 1>),
+            &nb, reinterpret_cast<const void *>(&cosenu_pair_stage<NW, SCHEME, 1, false>),
             64 * NW * TPB, 0) != hipSuccess)
         return 0;
     return nb * prop.multiProcessorCount * TPB;
@@ -374,7 +424,7 @@ int resident_teams_nw()
 int pair_supported(int VP, int bc) { return (VP == 64 || VP == 128 || VP == 256) && bc == 0; }
 
 hipError_t launch_stage_pair(const StageArgs &a1, const StageArgs &a2, int pair, int L,
-                             int nteams, hipStream_t s)
+                             int nteams, bool uniform, hipStream_t s)
 {
     if (!pair_supported(a1.VP, a1.bc) || a1.nz < 16 || (pair != 0 && pair != 1))
         return hipErrorInvalidValue;
@@ -384,8 +434,25 @@ hipError_t launch_stage_pair(const StageArgs &a1, const StageArgs &a2, int pair,
     } else if (L < 1 || a1.nz % L != 0) {
         return hipErrorInvalidValue;
     }
-    return a1.scheme == 0 ? launch_scheme<0>(a1, a2, pair, L, nteams, s)
-                          : launch_scheme<1>(a1, a2, pair, L, nteams, s);
+    if (uniform && a1.nvz != a1.VP)
+        return hipErrorInvalidValue;
+    return a1.scheme == 0 ? launch_scheme<0>(a1, a2, pair, L, nteams, uniform, s)
+                          : launch_scheme<1>(a1, a2, pair, L, nteams, uniform, s);
+}
+
+hipError_t launch_stage_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
+                                   int scheme, int L, int pair, bool uniform, hipStream_t s)
+{
+    if (!pair_supported(VP, 0) || L < 1 || (pair != 0 && pair != 1))
+        return hipErrorInvalidValue;
+    switch (VP) {
+    case 64: return scheme == 0 ? launch_batch_nw<2, 0>(runs, items, nitems, L, pair, uniform, s)
+                                : launch_batch_nw<2, 1>(runs, items, nitems, L, pair, uniform, s);
+    case 128: return scheme == 0 ? launch_batch_nw<4, 0>(runs, items, nitems, L, pair, uniform, s)
+                                 : launch_batch_nw<4, 1>(runs, items, nitems, L, pair, uniform, s);
+    default: return scheme == 0 ? launch_batch_nw<8, 0>(runs, items, nitems, L, pair, uniform, s)
+                                : launch_batch_nw<8, 1>(runs, items, nitems, L, pair, uniform, s);
+    }
 }
 
 int pair_resident_teams(int VP, int scheme)

@@ -210,8 +210,9 @@ typedef struct cosenu_batch_info {
     long long cells;      /* sum over runs of nz*nvz */
     long long steps_done;
     size_t device_bytes;
-    long long work_items; /* (run, z-chunk) items per stage */
+    long long work_items; /* (run, z-chunk) items per launch */
     int z_chunk;          /* chunk length of the first group */
+    int fast_variant;     /* 4: two RK4 stages per launch (default), 2: one */
 } cosenu_batch_info;
 
 int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out);

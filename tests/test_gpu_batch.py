@@ -2,8 +2,9 @@
 
 A batch of runs with uneven nz, nvz and physics must leave every run
 bit-identical to the same run advanced alone in its own context with the same
-FAST kernel variant (2: two lanes per velocity bin), and its diagnostics equal
-too.  Runs of one batch are grouped by (nvz rounded to 64, scheme).
+FAST kernel (variant 4: two stages per launch, the default; variant 2: one
+stage per launch, COSENU_BATCH_VARIANT=2), and its diagnostics equal too.
+Runs of one batch are grouped by (nvz rounded to 64, scheme).
 """
 import os
 
@@ -33,17 +34,22 @@ def _runs():
     return runs, ics
 
 
-@pytest.fixture
-def variant2(monkeypatch):
-    monkeypatch.setenv("COSENU_FAST_VARIANT", "2")
+@pytest.fixture(params=["2", "4"])
+def variant(request, monkeypatch):
+    monkeypatch.setenv("COSENU_FAST_VARIANT", request.param)
+    monkeypatch.setenv("COSENU_BATCH_VARIANT", request.param)
+    return int(request.param)
 
 
-def test_batch_runs_bit_identical_to_solo_runs(gpu, variant2):
+@pytest.mark.parametrize("nsteps", [20, 37])
+def test_batch_runs_bit_identical_to_solo_runs(gpu, variant, nsteps):
+    # 20 = one 16-step graph replay + 4 eager steps; 37 (odd) leaves the
+    # two-stage batches' states in their second buffers
     runs, ics = _runs()
-    nsteps = 20  # one 16-step graph replay + 4 eager steps
     with C.Batch(runs) as b:
         info = b.info()
         assert info["nruns"] == len(runs) and info["ngroups"] == 4
+        assert info["fast_variant"] == variant
         assert info["cells"] == sum(p.nz * p.nvz for p in runs)
         for r, (G, bG, s0) in enumerate(ics):
             b.set_profile(r, G, bG)
@@ -66,7 +72,7 @@ def test_batch_runs_bit_identical_to_solo_runs(gpu, variant2):
         assert not np.array_equal(solo, s0)
 
 
-def test_batch_matches_oracle(gpu, variant2):
+def test_batch_matches_oracle(gpu, variant):
     runs, ics = _runs()
     runs, ics = runs[:3], ics[:3]
     with C.Batch(runs) as b:

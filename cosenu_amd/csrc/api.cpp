@@ -537,7 +537,8 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
     const size_t n_state = (size_t)c->nz * 8 * c->VP;
     std::vector<double> vz, Hm;
     fill_grids(*p, c->d, c->VP, vz, Hm);
-    c->uniform = stream_supported(c->nvz, c->VP, COSENU_PERIODIC, vz.data()) != 0;
+    c->uniform = stream_supported(c->nvz, c->VP, COSENU_PERIODIC, vz.data()) != 0 &&
+                 env_int("COSENU_UNIFORM", 1) != 0;
     c->stream_ok = p->mode == COSENU_FAST && c->uniform && p->bc == COSENU_PERIODIC;
     if (p->mode == COSENU_FAST) {
         // z_chunk < 0 in the tune table = contiguous-range mode with that many

@@ -12,13 +12,19 @@
 // plus 4 on each side (recomputed by both neighbours), which the second stage
 // needs for its stencil.
 //
-// Bytes per cell and step: stages 1+2 read y (the window) and write Y2 and
-// Y3; stages 3+4 read Y3 (the window), y and Y2 and write the new state:
+// RK4 combination.  With Y2 = y + (dt/2) K1, Y3 = y + (dt/2) K2 and Y4 =
+// y + dt K3 (nuosc.hpp:320-348), the new state is
+//   y' = y + (dt/6)(K1 + 2 K2 + 2 K3 + K4) = P + Y4/3 + (dt/6) K4,
+//   P  = (Y2 + 2 Y3 - y)/3,
+// and P is formed by the stages 1+2 launch, which holds y, Y2 and Y3 of the
+// cell at once: stage 4 then needs no y.
+//
+// Bytes per cell and step: stages 1+2 read y (the window) and write P and Y3;
+// stages 3+4 read Y3 (the window), y (for Y4) and P and write the new state:
 // 64 + 128 + 192 + 64 = 448 B instead of 832 B for four separate stage
-// launches (plus the halo rows of the windows).  The second stage's pointwise
-// use of the first stage's input (y at stage 2, Y3 at stage 4) comes from the
-// row that has just left the window, kept in registers.  Stage 4 reads y a
-// second time, 4 rows after stage 3 did.
+// launches (plus the halo rows of the windows).  The second stage of stages
+// 1+2 takes y at its cell from the row that has just left the first window,
+// kept in registers.
 //
 // The new state is written to a second buffer (the stages 3+4 launch reads y
 // in neighbouring teams' halos while it runs), and the state buffers swap
@@ -217,17 +223,15 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         const bool face_b = jb >= j0 - 1 && jb < jend;
         const bool cell_b = jb >= j0 && jb < jend;
         const int ra_row = zrow(i), rb_row = zrow(jb);
-        // pointwise operands: y at the first stage's cell (stage 3), and y,
-        // Y2, Y3 at the second stage's cell (y for stage 2; all at stage 4).
-        // Row jb of the first stage's input (y for stages 1+2, Y3 for 3+4)
-        // left the window one iteration ago and is kept in `wprev`.
-        double ya[4], yb[4], y2b[4];
+        // pointwise operands: y at the first stage's cell for stage 3, P at
+        // the second stage's cell for stage 4; stage 2 takes y at its cell
+        // (row jb) from `wprev`, the row that left the first window one
+        // iteration ago.
+        double ya[4], pb[4];
         if (PAIR == 1 && cell_a)
             load4(A1.y, i, ya);
-        if (cell_b && PAIR == 1) {
-            load4(A2.y, jb, yb);
-            load4(A2.y2, jb, y2b);
-        }
+        if (cell_b && PAIR == 1)
+            load4(A2.y2, jb, pb); // P of the cell, written by stages 1+2
         if (coll)
             moments(ra, i + 4, wa[0][7], wa[1][7], wa[2][7], wa[3][7]);
         double F[4] = {0.0, 0.0, 0.0, 0.0};
@@ -245,12 +249,6 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
 #pragma unroll
             for (int f = 0; f < 4; f++)
                 oa[f] = (PAIR == 0 ? wa[f][3] : ya[f]) + A1.a * K[f];
-            if (PAIR == 0 && store && i >= j0 && i < jend) { // Y2, read by stage 4
-                double *o = A1.yout + (size_t)ra_row * zs + loff;
-#pragma unroll
-                for (int f = 0; f < 4; f++)
-                    o[(size_t)f * VP] = oa[f];
-            }
             if (coll)
                 moments(rb, i, oa[0], oa[1], oa[2], oa[3]);
         }
@@ -272,18 +270,20 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
                 rhs(wb, G, Fb, rb, jb, rb_row, K);
                 if (store) {
                     double *o = A2.yout + (size_t)rb_row * zs + loff;
+                    double *op = A1.yout + (size_t)rb_row * zs + loff; // P (stages 1+2)
 #pragma unroll
                     for (int f = 0; f < 4; f++) {
-                        double out;
                         if (PAIR == 0) {
-                            out = wprev[f] + A2.a * K[f];
-                        } else {
-                            const double y0 = yb[f];
-                            const double d =
-                                ((y2b[f] - y0) + 2.0 * (wprev[f] - y0)) + (wb[f][3] - y0);
-                            out = y0 + (d * (1.0 / 3.0) + A2.a * K[f]);
+                            // Y3 = y + (dt/2) K2; P = (Y2 + 2 Y3 - y)/3 with
+                            // y from the departed window row, Y2 = window centre
+                            const double y3 = wprev[f] + A2.a * K[f];
+                            o[(size_t)f * VP] = y3;
+                            op[(size_t)f * VP] =
+                                (fma(2.0, y3, wb[f][3]) - wprev[f]) * (1.0 / 3.0);
+                        } else { // y' = P + Y4/3 + (dt/6) K4
+                            o[(size_t)f * VP] =
+                                fma(A2.a, K[f], fma(wb[f][3], 1.0 / 3.0, pb[f]));
                         }
-                        o[(size_t)f * VP] = out;
                     }
                 }
             }

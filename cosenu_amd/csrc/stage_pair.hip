@@ -214,6 +214,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     }
     double Fa[4] = {0.0, 0.0, 0.0, 0.0}, Fb[4] = {0.0, 0.0, 0.0, 0.0};
     double wprev[4] = {0.0, 0.0, 0.0, 0.0}; // the row before the window
+    double yn[4] = {0.0, 0.0, 0.0, 0.0};    // stages 3+4: y of the next first-stage cell
 
     for (int i = ibeg; i < iend; i++) {
         double nxt[4];
@@ -227,11 +228,17 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         // the second stage's cell for stage 4; stage 2 takes y at its cell
         // (row jb) from `wprev`, the row that left the first window one
         // iteration ago.
+        // (y of stage 3's cell is loaded one iteration ahead, into yn)
         double ya[4], pb[4];
-        if (PAIR == 1 && cell_a)
-            load4(A1.y, i, ya);
-        if (cell_b && PAIR == 1)
-            load4(A2.y2, jb, pb); // P of the cell, written by stages 1+2
+        if (PAIR == 1) {
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                ya[f] = yn[f];
+            if (i + 1 >= j0 - 4 && i + 1 < jend + 4)
+                load4(A1.y, i + 1, yn);
+            if (cell_b)
+                load4(A2.y2, jb, pb); // P of the cell, written by stages 1+2
+        }
         if (coll)
             moments(ra, i + 4, wa[0][7], wa[1][7], wa[2][7], wa[3][7]);
         double F[4] = {0.0, 0.0, 0.0, 0.0};

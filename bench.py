@@ -27,11 +27,15 @@ sys.path.insert(0, ROOT)
 METRIC = "FP64 z·v cell-updates/s + HBM roofline %; surrogate vs solver P_ee sweep speedup"
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_GINST = 1024 * 2.4 / 4  # wave64 VALU instructions per ns at the peak clock
-# Bytes per cell of one RK4 step under the classic accumulator form (read
-# the stage input, y and the accumulator, write the accumulator and the next
-# stage input, 8 FP64 fields each): the figure the roofline is defined with.
-BYTES_PER_CELL_STEP_DEF = 1088
-# Algorithmic bytes per cell of each stage kernel, by FAST kernel variant:
+# Algorithmic bytes per cell-update (one cell, 8 FP64 fields, one RK4 step):
+# the classic accumulator form, per stage read the stage input, y and the
+# accumulator and write the accumulator and the next stage input -- stage 1
+# 64+128, stages 2 and 3 192+128, stage 4 192+64, 1088 in all.  The roofline
+# is defined with these figures: a launch's algorithmic bytes are those of the
+# stages it performs.
+ALG_STAGE_BYTES = (192, 320, 320, 256)
+BYTES_PER_CELL_STEP_DEF = sum(ALG_STAGE_BYTES)
+# Bytes per cell each stage kernel actually has to move, by FAST variant:
 #   1 (one lane per bin, classic form): 64+128, 192+128, 192+128, 192+64
 #   2 (two lanes per bin, stage-value form: no accumulator; stage 4 reads
 #      y, Y2, Y3 pointwise): 64+64, 128+64, 128+64, 256+64
@@ -39,8 +43,9 @@ BYTES_PER_CELL_STEP_DEF = 1088
 #      of the input read and of the output written: 2 + 2 bytes per cell
 #      (one 64-byte slot per z row and group of 32 bins, each way)
 #   4 (two stages per launch; the second stage's input stays in registers):
-#      stages 1+2 read y, write Y2 and Y3 (64+128); stages 3+4 read Y3, y, Y2,
-#      write the new state (192+64); the timing slots are the two launches
+#      stages 1+2 read y, write Y3 and P = (Y2 + 2 Y3 - y)/3 (64+128); stages
+#      3+4 read Y3, y, P, write the new state (192+64); the timing slots are
+#      the two launches
 STAGE_BYTES = {1: (192, 320, 320, 256), 2: (128, 192, 192, 320), 3: (132, 196, 196, 324),
                4: (192, 256, 0, 0)}
 
@@ -133,18 +138,20 @@ def main():
     # total time per step.
     variant = sim.info()["fast_variant"]
     sb = STAGE_BYTES[variant]
+    ab = ALG_STAGE_BYTES
     avg = [m / max(1, c) for m, c in zip(ms, cnt)]
+    # name: (avg ms per launch, algorithmic bytes/cell, moved bytes/cell, kernel, launches per step)
     if variant == 4:
-        kernels = {"stages1+2": (avg[0], sb[0], sim.stage_kernel_name(0), 1),
-                   "stages3+4": (avg[1], sb[1], sim.stage_kernel_name(1), 1)}
+        kernels = {"stages1+2": (avg[0], ab[0] + ab[1], sb[0], sim.stage_kernel_name(0), 1),
+                   "stages3+4": (avg[1], ab[2] + ab[3], sb[1], sim.stage_kernel_name(1), 1)}
     else:
-        kernels = {"stage1": (avg[0], sb[0], sim.stage_kernel_name(0), 1),
-                   "stage2+3": ((ms[1] + ms[2]) / max(1, cnt[1] + cnt[2]), sb[1],
+        kernels = {"stage1": (avg[0], ab[0], sb[0], sim.stage_kernel_name(0), 1),
+                   "stage2+3": ((ms[1] + ms[2]) / max(1, cnt[1] + cnt[2]), ab[1], sb[1],
                                 sim.stage_kernel_name(1), 2),
-                   "stage4": (avg[3], sb[3], sim.stage_kernel_name(3), 1)}
-    dom = max(kernels, key=lambda k: kernels[k][0] * kernels[k][3])
-    dom_ms, dom_bytes, dom_name, _ = kernels[dom]
-    achieved = dom_bytes * cells / (dom_ms * 1e-3) / 1e9
+                   "stage4": (avg[3], ab[3], sb[3], sim.stage_kernel_name(3), 1)}
+    dom = max(kernels, key=lambda k: kernels[k][0] * kernels[k][4])
+    dom_ms, dom_alg, dom_moved, dom_name, _ = kernels[dom]
+    achieved = dom_alg * cells / (dom_ms * 1e-3) / 1e9
     step_kernel_ms = sum(avg)
     step_gbs = sum(sb) * cells / (step_kernel_ms * 1e-3) / 1e9
     # HBM bytes per launch of the dominant kernel, from the rocprofv3 FETCH_SIZE /
@@ -196,12 +203,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": dom_name, "kernel_stage": dom,
-                     "kernel_bytes_per_cell": dom_bytes, "kernel_avg_ms": dom_ms,
+                     "kernel_alg_bytes_per_cell": dom_alg, "kernel_avg_ms": dom_ms,
+                     "kernel_moved_bytes_per_cell": dom_moved,
+                     "kernel_moved_GBs": dom_moved * cells / (dom_ms * 1e-3) / 1e9,
                      "fast_variant": variant,
-                     "stage_bytes_per_cell": list(sb),
+                     "stage_moved_bytes_per_cell": list(sb),
                      "stage_avg_ms": avg,
-                     "step_kernels_GBs": step_gbs,
-                     "step_kernels_frac": step_gbs / HBM_PEAK_GBS,
+                     "step_moved_GBs": step_gbs,
                      "frac_at_1088B_per_cell_update": value * BYTES_PER_CELL_STEP_DEF / 1e9 / HBM_PEAK_GBS,
                      "valu": valu},
     }

@@ -331,6 +331,7 @@ struct cosenu_ctx {
     size_t bytes = 0;
     hipStream_t stream = nullptr, copy_stream = nullptr;
     double *y = nullptr, *acc = nullptr, *A = nullptr, *B = nullptr, *y0 = nullptr;
+    double *states = nullptr; // one allocation holding y, acc, A, B, y0
     double *prof = nullptr, *vz = nullptr, *Hm = nullptr;
     double *diag_part = nullptr, *diag_out = nullptr, *snap = nullptr;
     int npart = 0;
@@ -370,8 +371,7 @@ int dev_alloc(size_t &bytes, T **p, size_t n)
 
 void release(cosenu_ctx *c)
 {
-    double *bufs[] = {c->y, c->acc, c->A, c->B, c->y0, c->prof, c->vz, c->Hm,
-                      c->diag_part, c->diag_out, c->snap};
+    double *bufs[] = {c->states, c->prof, c->vz, c->Hm, c->diag_part, c->diag_out, c->snap};
     for (double *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -580,11 +580,15 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
         if (!rc)
             rc = r;
     };
-    chk(dev_alloc(c->bytes, &c->y, n_state));
-    if (!rc) chk(dev_alloc(c->bytes, &c->acc, n_state));
-    if (!rc) chk(dev_alloc(c->bytes, &c->A, n_state));
-    if (!rc) chk(dev_alloc(c->bytes, &c->B, n_state));
-    if (!rc) chk(dev_alloc(c->bytes, &c->y0, n_state));
+    // the five states in one allocation
+    chk(dev_alloc(c->bytes, &c->states, 5 * n_state));
+    if (!rc) {
+        c->y = c->states;
+        c->acc = c->states + n_state;
+        c->A = c->states + 2 * n_state;
+        c->B = c->states + 3 * n_state;
+        c->y0 = c->states + 4 * n_state;
+    }
     if (!rc) chk(dev_alloc(c->bytes, &c->prof, (size_t)c->nz * 2 * c->VP));
     if (!rc) chk(dev_alloc(c->bytes, &c->vz, (size_t)c->VP));
     if (!rc) chk(dev_alloc(c->bytes, &c->Hm, (size_t)c->nz));

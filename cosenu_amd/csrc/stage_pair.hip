@@ -212,6 +212,13 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         moments(ra, ibeg + 2, wa[0][5], wa[1][5], wa[2][5], wa[3][5]);
         moments(ra, ibeg + 3, wa[0][6], wa[1][6], wa[2][6], wa[3][6]);
     }
+    // The window's first rows have landed before the loop starts.  Without
+    // this wait the compiler's wait-count analysis, which merges the loop
+    // entry with the back edge, takes the last of them as possibly in flight
+    // inside the loop and, as the y and P loads of an iteration are
+    // conditional, waits for all but 4 of the iteration's loads right after
+    // issuing them (a memory latency per iteration in stages 3+4).
+    __builtin_amdgcn_s_waitcnt(0);
     double Fa[4] = {0.0, 0.0, 0.0, 0.0}, Fb[4] = {0.0, 0.0, 0.0, 0.0};
     double wprev[4] = {0.0, 0.0, 0.0, 0.0}; // the row before the window
     double yn[4] = {0.0, 0.0, 0.0, 0.0};    // stages 3+4: y of the next first-stage cell

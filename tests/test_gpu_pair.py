@@ -80,9 +80,14 @@ def test_pair_graph_eager_and_call_pattern_identical(gpu, monkeypatch):
         assert np.array_equal(outs[0], o)
 
 
-@pytest.mark.parametrize("chunk", ["24", "-1", "-2"])
-def test_pair_launch_shapes_identical(gpu, monkeypatch, chunk):
-    po, pc = _pair(O.FV, 480, 128)
+@pytest.mark.parametrize("chunk", ["24", "15", "-1", "-2"])
+@pytest.mark.parametrize("scheme", [O.FV, O.FD])
+def test_pair_launch_shapes_identical(gpu, monkeypatch, scheme, chunk):
+    """Chunks of 48 rows against chunks of 24 and 15 rows and contiguous
+    ranges: bit for bit.  (FD runs two z steps per loop trip; chunks of 48
+    rows give an odd step count, which ends with an idle step, 15 rows an
+    even one.)"""
+    po, pc = _pair(scheme, 480, 128)
     _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=34, amp=1e-3)
     outs = []
     for ch in ("48", chunk):

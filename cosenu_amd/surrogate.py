@@ -212,6 +212,11 @@ def solve(elns: list, setup: SolverSetup, log=None) -> tuple:
             acc = np.zeros((len(chunk), NV))
             nsamp = 0
             done = 0
+            while log and first - done > 1000:  # progress lines on long runs
+                B.step(1000)
+                done += 1000
+                B.sync()
+                log(f"solver: runs {b0}..{b0 + len(chunk)}: step {done}")
             for target in range(first, setup.steps + 1, setup.sample_every):
                 B.step(target - done)
                 done = target

@@ -17,6 +17,18 @@
    (predicting the training-set mean P_ee(v)).
 
 Prints one JSON line and writes it (plus the ensemble weights) to --out.
+
+The dataset can be produced in pieces (one GPU job each):
+
+    python -m cosenu_amd.train_surrogate --nz 8192 --steps 8000 --runs 2048 \
+        --data-dir D --gen 0 1024
+    ... --gen 1024 2048
+    python -m cosenu_amd.train_surrogate --nz 8192 --steps 8000 --runs 2048 --data-dir D
+
+Run i of the sample is the same whatever the sample size, so a piece
+`data_<a>_<b>.npz` holds runs [a, b) of every sample with the same seed; the
+full job loads the pieces it finds (same grid, schedule and seed) and solves
+what is missing.
 """
 from __future__ import annotations
 
@@ -35,6 +47,63 @@ def log(msg):
     print(msg, flush=True)
 
 
+def _meta(args, setup):
+    return {"nz": setup.nz, "z0": setup.z0, "z1": setup.z1, "steps": setup.steps,
+            "avg_frac": setup.avg_frac, "sample_every": setup.sample_every, "nvz": S.NV,
+            "seed": args.seed}
+
+
+def save_piece(args, setup, a, b):
+    """Solve runs [a, b) of the sample and write them to --data-dir."""
+    elns = S.sample_elns(b, args.seed)[a:b]
+    X = np.stack([S.features(e) for e in elns])
+    Y, t = S.solve(elns, setup, log=log)
+    os.makedirs(args.data_dir, exist_ok=True)
+    path = os.path.join(args.data_dir, f"data_{a}_{b}.npz")
+    np.savez_compressed(path, X=X, Y=Y, a=a, b=b, solver_s=t,
+                        meta=json.dumps(_meta(args, setup)))
+    log(json.dumps({"piece": [a, b], "solver_s": t, "file": path}))
+
+
+def dataset(args, setup, elns):
+    """Targets of runs [0, len(elns)): pieces found in --data-dir with the same
+    grid, schedule and seed are loaded, the rest solved.
+    Returns (Y, solver seconds, [[a, b, source], ...])."""
+    n = len(elns)
+    Y = np.full((n, S.NV), np.nan, dtype=np.float32)
+    have = np.zeros(n, dtype=bool)
+    t = 0.0
+    pieces = []
+    meta = _meta(args, setup)
+    if args.data_dir and os.path.isdir(args.data_dir):
+        for name in sorted(os.listdir(args.data_dir)):
+            if not (name.startswith("data_") and name.endswith(".npz")):
+                continue
+            with np.load(os.path.join(args.data_dir, name)) as z:
+                if json.loads(str(z["meta"])) != meta:
+                    continue
+                a, b = int(z["a"]), int(z["b"])
+                if a >= n:
+                    continue
+                m = min(b, n) - a
+                X = np.stack([S.features(e) for e in elns[a:a + m]])
+                if not np.array_equal(X, z["X"][:m]):
+                    raise SystemExit(f"{name}: features differ from the sample")
+                Y[a:a + m] = z["Y"][:m]
+                have[a:a + m] = True
+                t += float(z["solver_s"]) * m / (b - a)
+                pieces.append([a, a + m, name])
+    todo = np.nonzero(~have)[0]
+    if todo.size:
+        Yt, tt = S.solve([elns[i] for i in todo], setup, log=log)
+        Y[todo] = Yt
+        t += tt
+        cut = np.nonzero(np.diff(todo) > 1)[0] + 1
+        for seg in np.split(todo, cut):
+            pieces.append([int(seg[0]), int(seg[-1]) + 1, "solved here"])
+    return Y, t, pieces
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--runs", type=int, default=2000)
@@ -47,13 +116,22 @@ def main(argv=None):
     ap.add_argument("--bound", type=float, default=0.02)
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--out", default="")
+    ap.add_argument("--data-dir", default="", help="directory of dataset pieces data_<a>_<b>.npz")
+    ap.add_argument("--gen", type=int, nargs=2, metavar=("A", "B"),
+                    help="solve runs [A, B) of the sample into --data-dir and exit")
     args = ap.parse_args(argv)
 
     setup = S.SolverSetup(nz=args.nz, z0=-args.zhalf, z1=args.zhalf, steps=args.steps)
     t0 = time.perf_counter()
+    if args.gen:
+        a, b = args.gen
+        if not args.data_dir or not 0 <= a < b:
+            ap.error("--gen A B needs --data-dir and 0 <= A < B")
+        save_piece(args, setup, a, b)
+        return 0
     elns = S.sample_elns(args.runs, args.seed)
     X = np.stack([S.features(e) for e in elns])
-    Y, t_data = S.solve(elns, setup, log=log)
+    Y, t_data, pieces = dataset(args, setup, elns)
     log(f"dataset: {args.runs} runs, solver {t_data:.1f} s")
     n_train = int(round(0.9 * args.runs))
     Xtr, Ytr, Xte, Yte = X[:n_train], Y[:n_train], X[n_train:], Y[n_train:]
@@ -106,7 +184,9 @@ def main(argv=None):
         "no_network_baseline_heldout_mean_abs_dPee": base_err,
         "dataset": {"runs": args.runs, "train": n_train, "heldout": args.runs - n_train,
                     "nz": setup.nz, "nvz": S.NV, "z": [setup.z0, setup.z1], "steps": setup.steps,
-                    "avg_over_final_fraction": setup.avg_frac, "solver_s": t_data},
+                    "avg_over_final_fraction": setup.avg_frac,
+                    "sample_every_steps": setup.sample_every, "solver_s": t_data,
+                    "pieces": pieces},
         "training": {"members": cfg.members, "steps": cfg.steps, "batch": cfg.batch, "lr": cfg.lr,
                      "weight_decay": cfg.weight_decay, "seconds": t_train,
                      "loss_curves": curves},

@@ -294,14 +294,25 @@ void tune_store(int VP, int scheme, int nz, TuneVal v)
 }
 
 // Contiguous-range mode: `mult` times the resident teams (one team = the
-// wavefronts covering all bins of a z range), at most one team per 8 z cells.
+// wavefronts covering all bins of a z range), at most one team per 2 z cells
+// (8 for the barrier-free kernel).
+//
+// Two-stage launches of small problems: a team's launch time is about
+// (rows + 9) z steps (the first stage's halo), each step slower with two
+// wavefronts per SIMD than with one (x1.44 FV, x1.23 FD measured at nz =
+// 1024, nvz = 64).  Where the resident teams would get few rows each, half
+// as many teams (one wavefront per SIMD) finish sooner: below 8 rows per
+// resident team for FV, 3 for FD.  (nz = 1024, nvz = 64: 512 teams of 2
+// rows, 63 us per FV step against 90 us with 128 teams of 8 rows.)
 int range_teams(int nz, int VP, int scheme, int bc, int variant, int mult)
 {
     long t = variant == 3   ? (long)stream_resident_waves(scheme) / (VP / 32)
              : variant == 4 ? (long)pair_resident_teams(VP, scheme)
                             : (long)fast_resident_teams(VP, scheme, bc);
     t *= mult;
-    t = std::min<long>(t, nz / 8);
+    if (variant == 4 && mult == 1 && t >= 2 && (nz + t - 1) / t < (scheme == 0 ? 8 : 3))
+        t /= 2;
+    t = std::min<long>(t, variant == 3 ? nz / 8 : nz / 2);
     return (int)std::max<long>(t, 1);
 }
 
@@ -973,9 +984,9 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = fail(COSENU_EHIP, "cosenu_autotune: hipEventCreate failed");
-    // chunk lengths, then (split-lane only) contiguous ranges with 1 or 2
-    // times the resident teams, encoded as chunk = -1, -2
-    const int cands[] = {8, 16, 32, 64, 128, 256, -1, -2};
+    // chunk lengths, then (variants 2-4) contiguous ranges with 1 or 2 times
+    // the resident teams, encoded as chunk = -1, -2
+    const int cands[] = {1, 2, 4, 8, 16, 32, 64, 128, 256, -1, -2};
     for (int var = q.bc == COSENU_OPEN ? 2 : 1; var <= 4 && !rc; var++)
         for (int L : cands) {
             if (rc || (L > 0 && (L > c->nz || c->nz % L)) || (L < 0 && var == 1) ||

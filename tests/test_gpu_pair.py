@@ -80,10 +80,10 @@ def test_pair_graph_eager_and_call_pattern_identical(gpu, monkeypatch):
         assert np.array_equal(outs[0], o)
 
 
-@pytest.mark.parametrize("chunk", ["24", "15", "-1", "-2"])
+@pytest.mark.parametrize("chunk", ["24", "15", "2", "1", "-1", "-2"])
 @pytest.mark.parametrize("scheme", [O.FV, O.FD])
 def test_pair_launch_shapes_identical(gpu, monkeypatch, scheme, chunk):
-    """Chunks of 48 rows against chunks of 24 and 15 rows and contiguous
+    """Chunks of 48 rows against chunks of 24, 15, 2 and 1 rows and contiguous
     ranges: bit for bit.  (FD runs two z steps per loop trip; chunks of 48
     rows give an odd step count, which ends with an idle step, 15 rows an
     even one.)"""
@@ -130,3 +130,21 @@ def test_pair_state_io_between_odd_calls(gpu):
         sim.step(7)
         want = sim.download()
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("scheme", [O.FV, O.FD])
+def test_pair_small_problem_default_launch(gpu, scheme):
+    """A problem too small to fill the device (nz = 1024, nvz = 64, the
+    CPU-sized case's grid) gets ranges of 2 rows, one wavefront per SIMD;
+    its steps match the oracle."""
+    po, pc = _pair(scheme, 1024, 64)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=37, amp=1e-3)
+    ref = O.step(po, s, 6)
+    with C.Simulation(pc) as sim:
+        info = sim.info()
+        assert info["fast_variant"] == 4
+        assert info["range_teams"] == 512 and info["z_chunk"] == 2
+        sim.upload(s)
+        sim.step(6)
+        out = sim.download()
+    assert _rel(out, ref) <= 1e-10

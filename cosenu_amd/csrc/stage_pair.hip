@@ -14,17 +14,22 @@
 //
 // RK4 combination.  With Y2 = y + (dt/2) K1, Y3 = y + (dt/2) K2 and Y4 =
 // y + dt K3 (nuosc.hpp:320-348), the new state is
-//   y' = y + (dt/6)(K1 + 2 K2 + 2 K3 + K4) = P + Y4/3 + (dt/6) K4,
-//   P  = (Y2 + 2 Y3 - y)/3,
-// and P is formed by the stages 1+2 launch, which holds y, Y2 and Y3 of the
-// cell at once: stage 4 then needs no y.
+//   y' = y + (dt/6)(K1 + 2 K2 + 2 K3 + K4) = y + ((Q + (Y4 - y))/3 + (dt/6) K4),
+//   Q  = (Y2 - y) + 2 (Y3 - y),
+// the stage-value form of the split-lane kernel.  Q is formed by the stages
+// 1+2 launch, which holds y, Y2 and Y3 of the cell at once; the stages 3+4
+// launch reads y at the first stage's cell (for Y4) and keeps it in a
+// wavefront-private LDS ring until the second stage reaches that cell 4 z
+// steps later.  Only the last addition rounds a state-sized value, as in the
+// reference's update: a form that rounds state-sized products (such as
+// (Y2 + 2 Y3 - y)/3 + Y4/3) drifts the conserved sums by ~1e-17 per step.
 //
-// Bytes per cell and step: stages 1+2 read y (the window) and write P and Y3;
-// stages 3+4 read Y3 (the window), y (for Y4) and P and write the new state:
-// 64 + 128 + 192 + 64 = 448 B instead of 832 B for four separate stage
-// launches (plus the halo rows of the windows).  The second stage of stages
-// 1+2 takes y at its cell from the row that has just left the first window,
-// kept in registers.
+// Bytes per cell and step: stages 1+2 read y (the window) and write Q and Y3;
+// stages 3+4 read Y3 (the window), y and Q and write the new state: 64 + 128
+// + 192 + 64 = 448 B instead of 832 B for four separate stage launches (plus
+// the halo rows of the windows).  The second stage of stages 1+2 takes y at
+// its cell from the row that has just left the first window, kept in
+// registers.
 //
 // The new state is written to a second buffer (the stages 3+4 launch reads y
 // in neighbouring teams' halos while it runs), and the state buffers swap
@@ -51,6 +56,10 @@ using namespace fastk;
 constexpr int kRing = 16, kSync = 4;
 static_assert(kRing - 5 >= kSync && kSync <= 4, "moment ring too shallow for the barrier spacing");
 
+// stages 3+4: y of the first stage's last 8 cells, per wavefront
+// [slot = row % 8][field][lane]
+constexpr int kYRing = 8 * 4 * 64;
+
 // A1: the first stage of the pair (1 or 3), A2: the second (2 or 4) of the
 // same run.  PAIR 0: stages 1+2, PAIR 1: stages 3+4.  UNI: no padding bins
 // and every wavefront's bins of one sign of v (the common case): no masks
@@ -58,8 +67,8 @@ static_assert(kRing - 5 >= kSync && kSync <= 4, "moment ring too shallow for the
 template <int NW, int SCHEME, int PAIR, bool UNI>
 __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &A2, int L, int j0,
                                           int jend, bool active, double (*ra)[NW][8],
-                                          double (*rb)[NW][8])
-// ra, rb: [kRing][NW][8]
+                                          double (*rb)[NW][8], double *yring)
+// ra, rb: [kRing][NW][8]; yring: the team's NW x kYRing doubles (stages 3+4)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wit = wave % NW;
@@ -78,6 +87,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     const bool live = UNI || v < A1.nvz;
     const bool store = active && live;
     const int loff = 4 * h * VP + v; // this lane's first field within a z row
+    double *const yr = yring + wit * kYRing; // this wavefront's part of the y ring
 
     auto zrow = [&](int m) -> int {
         int w = m < 0 ? m + nz : m; // m >= -9 > -nz
@@ -215,7 +225,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     // The window's first rows have landed before the loop starts.  Without
     // this wait the compiler's wait-count analysis, which merges the loop
     // entry with the back edge, takes the last of them as possibly in flight
-    // inside the loop and, as the y and P loads of an iteration are
+    // inside the loop and, as the y and Q loads of an iteration are
     // conditional, waits for all but 4 of the iteration's loads right after
     // issuing them (a memory latency per iteration in stages 3+4).
     __builtin_amdgcn_s_waitcnt(0);
@@ -231,24 +241,26 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         const bool face_b = jb >= j0 - 1 && jb < jend;
         const bool cell_b = jb >= j0 && jb < jend;
         const int ra_row = zrow(i), rb_row = zrow(jb);
-        // pointwise operands: y at the first stage's cell for stage 3, P at
-        // the second stage's cell for stage 4; stage 2 takes y at its cell
-        // (row jb) from `wprev`, the row that left the first window one
-        // iteration ago.
-        // (y of stage 3's cell is loaded one iteration ahead, into yn)
-        // (pb is defined on every path: with an undefined value on the path
+        // pointwise operands: y at the first stage's cell for stage 3 (loaded
+        // one iteration ahead, into yn, and kept in the LDS ring for stage
+        // 4), Q at the second stage's cell for stage 4; stage 2 takes y at
+        // its cell (row jb) from `wprev`, the row that left the first window
+        // one iteration ago.
+        // (qb is defined on every path: with an undefined value on the path
         // that skips its load, the optimiser folds the stage-4 combination
         // into the load's branch, and the wait for the load follows it at
         // once instead of at the end of the iteration)
-        double ya[4], pb[4] = {0.0, 0.0, 0.0, 0.0};
+        double ya[4], qb[4] = {0.0, 0.0, 0.0, 0.0};
         if (PAIR == 1) {
 #pragma unroll
-            for (int f = 0; f < 4; f++)
+            for (int f = 0; f < 4; f++) {
                 ya[f] = yn[f];
+                yr[((i & 7) * 4 + f) * 64 + lane] = ya[f];
+            }
             if (i + 1 >= j0 - 4 && i + 1 < jend + 4)
                 load4(A1.y, i + 1, yn);
             if (cell_b)
-                load4(A2.y2, jb, pb); // P of the cell, written by stages 1+2
+                load4(A2.y2, jb, qb); // Q of the cell, written by stages 1+2
         }
         if (coll)
             moments(ra, i + 4, wa[0][7], wa[1][7], wa[2][7], wa[3][7]);
@@ -288,19 +300,21 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
                 rhs(wb, G, Fb, rb, jb, rb_row, K);
                 if (store) {
                     double *o = A2.yout + (size_t)rb_row * zs + loff;
-                    double *op = A1.yout + (size_t)rb_row * zs + loff; // P (stages 1+2)
+                    double *oq = A1.yout + (size_t)rb_row * zs + loff; // Q (stages 1+2)
 #pragma unroll
                     for (int f = 0; f < 4; f++) {
                         if (PAIR == 0) {
-                            // Y3 = y + (dt/2) K2; P = (Y2 + 2 Y3 - y)/3 with
-                            // y from the departed window row, Y2 = window centre
-                            const double y3 = wprev[f] + A2.a * K[f];
+                            // Y3 = y + (dt/2) K2; Q = (Y2 - y) + 2 (Y3 - y)
+                            // with y from the departed window row, Y2 = the
+                            // window centre
+                            const double y0 = wprev[f];
+                            const double y3 = y0 + A2.a * K[f];
                             o[(size_t)f * VP] = y3;
-                            op[(size_t)f * VP] =
-                                (fma(2.0, y3, wb[f][3]) - wprev[f]) * (1.0 / 3.0);
-                        } else { // y' = P + Y4/3 + (dt/6) K4
-                            o[(size_t)f * VP] =
-                                fma(A2.a, K[f], fma(wb[f][3], 1.0 / 3.0, pb[f]));
+                            oq[(size_t)f * VP] = (wb[f][3] - y0) + 2.0 * (y3 - y0);
+                        } else { // y' = y + ((Q + (Y4 - y))/3 + (dt/6) K4)
+                            const double y0 = yr[((jb & 7) * 4 + f) * 64 + lane];
+                            const double d = qb[f] + (wb[f][3] - y0);
+                            o[(size_t)f * VP] = y0 + (d * (1.0 / 3.0) + A2.a * K[f]);
                         }
                     }
                 }
@@ -341,17 +355,20 @@ void cosenu_pair_stage(StageArgs A1, StageArgs A2, int L, int nchunks, int nteam
     constexpr int TPB = NW >= 4 ? 1 : 4 / NW; // teams per block
     __shared__ double ring_a[TPB][kRing][NW][8];
     __shared__ double ring_b[TPB][kRing][NW][8];
+    __shared__ double yring[PAIR == 1 ? TPB * NW * kYRing : 1];
     const int team = (threadIdx.x >> 6) / NW;
     const int g = xcd_block_order() * TPB + team;
+    double *const yr = yring + (PAIR == 1 ? team * NW * kYRing : 0);
     if (nteams > 0) {
         const bool active = g < nteams;
         const int j0 = active ? (int)(((long)g * A1.nz) / nteams) : 0;
         const int j1 = active ? (int)(((long)(g + 1) * A1.nz) / nteams) : 0;
-        pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, j0, j1, active, ring_a[team], ring_b[team]);
+        pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, j0, j1, active, ring_a[team], ring_b[team], yr);
     } else {
         const bool active = g < nchunks;
         const int j0 = (active ? g : 0) * L;
-        pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, j0, j0 + L, active, ring_a[team], ring_b[team]);
+        pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, j0, j0 + L, active, ring_a[team], ring_b[team],
+                                         yr);
     }
 }
 
@@ -366,6 +383,7 @@ void cosenu_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int 
     constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
     __shared__ double ring_a[TPB][kRing][NW][8];
     __shared__ double ring_b[TPB][kRing][NW][8];
+    __shared__ double yring[PAIR == 1 ? TPB * NW * kYRing : 1];
     const int team = (threadIdx.x >> 6) / NW;
     const int item = xcd_block_order() * TPB + team;
     const bool active = item < nitems;
@@ -375,7 +393,8 @@ void cosenu_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int 
     StageArgs A2 = batch_stage_args(r, PAIR == 0 ? 2 : 4, true);
     if (PAIR == 1)
         A2.yout = r.acc;
-    pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, it.y, it.y + L, active, ring_a[team], ring_b[team]);
+    pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, it.y, it.y + L, active, ring_a[team], ring_b[team],
+                                     yring + (PAIR == 1 ? team * NW * kYRing : 0));
 }
 
 template <int NW, int SCHEME>

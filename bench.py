@@ -43,9 +43,9 @@ BYTES_PER_CELL_STEP_DEF = sum(ALG_STAGE_BYTES)
 #      of the input read and of the output written: 2 + 2 bytes per cell
 #      (one 64-byte slot per z row and group of 32 bins, each way)
 #   4 (two stages per launch; the second stage's input stays in registers):
-#      stages 1+2 read y, write Y3 and P = (Y2 + 2 Y3 - y)/3 (64+128); stages
-#      3+4 read Y3, y, P, write the new state (192+64); the timing slots are
-#      the two launches
+#      stages 1+2 read y, write Y3 and Q = (Y2 - y) + 2 (Y3 - y) (64+128);
+#      stages 3+4 read Y3, y, Q, write the new state (192+64); the timing
+#      slots are the two launches
 STAGE_BYTES = {1: (192, 320, 320, 256), 2: (128, 192, 192, 320), 3: (132, 196, 196, 324),
                4: (192, 256, 0, 0)}
 

@@ -182,6 +182,22 @@ def main():
             valu = {"wave_insts_per_launch": rec[dom], "achieved": ginst, "peak": VALU_PEAK_GINST,
                     "unit": "G wave-instructions/s", "frac": ginst / VALU_PEAK_GINST}
 
+    # Device-copy bandwidth on the same device: one state (nz x 8 x VP doubles)
+    # copied between the context's two state buffers (hipMemcpyAsync, read +
+    # write), less the per-call synchronisation measured by a same-buffer call.
+    sim.copy_state(C.STATE, C.STATE0)
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.copy_state(C.STATE, C.STATE)  # no copy: the call's synchronisation only
+    t_sync = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.copy_state(C.STATE, C.STATE0)
+    t_copy = (time.perf_counter() - t0) / reps - t_sync
+    state_bytes = p.nz * 8 * ((p.nvz + 63) // 64 * 64) * 8
+    copy_gbs = 2 * state_bytes / t_copy / 1e9 if t_copy > 0 else None
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -211,6 +227,10 @@ def main():
                      "stage_avg_ms": avg,
                      "step_moved_GBs": step_gbs,
                      "frac_at_1088B_per_cell_update": value * BYTES_PER_CELL_STEP_DEF / 1e9 / HBM_PEAK_GBS,
+                     "copy_GBs": copy_gbs,
+                     "frac_of_copy": achieved / copy_gbs if copy_gbs else None,
+                     "moved_frac_of_copy": (dom_moved * cells / (dom_ms * 1e-3) / 1e9 / copy_gbs
+                                            if copy_gbs else None),
                      "valu": valu},
     }
     sim.close()

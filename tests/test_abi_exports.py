@@ -1,0 +1,55 @@
+"""The C-ABI library loads without a GPU and exports every function that
+include/cosenu_hip.h declares (no compute calls: only the ABI version)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.join(os.path.dirname(__file__), "..")
+HEADER = os.path.join(ROOT, "include", "cosenu_hip.h")
+LIB = os.path.join(ROOT, "cosenu_amd", "libcosenu_hip.so")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", " ", src, flags=re.S)  # comments
+    src = re.sub(r"//[^\n]*", " ", src)
+    names = re.findall(r"\b(cosenu_\w+)\s*\(", src)
+    # typedefs / struct tags are not followed by '(' at file scope; keep the
+    # function names in declaration order, once each
+    seen = []
+    for n in names:
+        if n not in seen:
+            seen.append(n)
+    return seen
+
+
+def test_header_declares_the_entry_points():
+    names = declared()
+    for must in ("cosenu_create", "cosenu_destroy", "cosenu_step", "cosenu_rhs",
+                 "cosenu_diagnostics", "cosenu_batch_create", "cosenu_autotune",
+                 "cosenu_last_error", "cosenu_abi_version"):
+        assert must in names, must
+    assert len(names) >= 40
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_library_exports_every_declared_function():
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    missing = [n for n in declared() if n not in exported]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_library_loads_without_a_gpu():
+    L = ctypes.CDLL(LIB)
+    for n in declared():
+        getattr(L, n)  # AttributeError if the dynamic loader cannot resolve it
+    L.cosenu_abi_version.restype = ctypes.c_int
+    text = open(HEADER).read()
+    m = re.search(r"#define\s+COSENU_ABI_VERSION\s+(\d+)", text)
+    assert m and L.cosenu_abi_version() == int(m.group(1))

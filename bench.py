@@ -102,8 +102,8 @@ def main():
     ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c4"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the solver would shard as z-slabs with a periodic halo exchange; see DESIGN.md).",
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(the solver would shard as z-slabs with a periodic halo exchange).",
               file=sys.stderr)
         sys.exit(2)
 

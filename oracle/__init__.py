@@ -1,4 +1,4 @@
-"""CPU checker for the HIP library -- TEST INFRASTRUCTURE ONLY.
+"""CPU restatement the HIP library is tested against -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
 import this package.  The product (``cosenu_amd``) never does.

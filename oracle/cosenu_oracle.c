@@ -1,7 +1,7 @@
 /*
  * cosenu_oracle.c -- CPU restatement of COSEnu's time-step path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library:
+ * TEST INFRASTRUCTURE ONLY.  The HIP library is tested against this file:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
  * it.  The product path never calls it.
  *

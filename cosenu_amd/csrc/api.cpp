@@ -1138,9 +1138,9 @@ int batch_enqueue_step(cosenu_batch *b, hipStream_t s)
 // Largest tuned (or default) chunk that divides every nz of the group.
 int group_chunk(const cosenu_batch *b, const Group &g)
 {
-    int gcd = 0;
+    int maxnz = 1;
     for (int r : g.runs)
-        gcd = std::gcd(gcd, b->P[r].nz);
+        maxnz = std::max(maxnz, b->P[r].nz);
     int want = env_int("COSENU_BATCH_ZCHUNK", 0);
     TuneVal tv;
     if (want <= 0 && tune_lookup(g.VP, g.scheme, -1, &tv))
@@ -1161,10 +1161,9 @@ int group_chunk(const cosenu_batch *b, const Group &g)
             want = (int)std::max(8L, std::min(32L, rows / target));
         }
     }
-    int L = std::min(want, gcd);
-    while (L > 1 && gcd % L)
-        L--;
-    return std::max(L, 1);
+    // chunks need not divide a run's nz: a run's last work item takes the
+    // remaining rows (the kernels take ranges shorter than the chunk)
+    return std::max(1, std::min(want, maxnz));
 }
 
 } // namespace

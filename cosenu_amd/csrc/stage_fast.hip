@@ -558,7 +558,8 @@ __global__ __launch_bounds__(NW >= 8 ? 512 : 256) void cosenu_fast2_batch(const 
     const bool active = item < nitems;
     const int2 it = items[active ? item : 0];
     const StageArgs A = batch_stage_args(runs[it.x], stage, true);
-    fast2_body<NW, SCHEME, STAGE, false>(A, L, it.y, it.y + L, active, ring[team]);
+    // a run's last item may be shorter than L (its nz need not be a multiple)
+    fast2_body<NW, SCHEME, STAGE, false>(A, L, it.y, min(it.y + L, A.nz), active, ring[team]);
 }
 
 template <int NW, int SCHEME, bool OPENB>

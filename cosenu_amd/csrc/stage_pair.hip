@@ -387,7 +387,8 @@ void cosenu_pair_stage(StageArgs A1, StageArgs A2, int L, int nchunks, int nteam
 }
 
 // Batched launch: work item k = (run, first z cell) of runs with the same VP
-// and scheme, all with chunk length L.  `runs` holds each run's current state
+// and scheme, chunks of L rows (a run's last one takes what remains of its
+// nz; every team runs the same L + 9 z steps).  `runs` holds each run's current state
 // in .y and the spare in .acc (the host passes the array with the two
 // swapped after every step).
 template <int NW, int SCHEME, int PAIR, bool UNI>
@@ -407,7 +408,9 @@ void cosenu_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int 
     StageArgs A2 = batch_stage_args(r, PAIR == 0 ? 2 : 4, true);
     if (PAIR == 1)
         A2.yout = r.acc;
-    pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, it.y, it.y + L, active, ring_a[team], ring_b[team],
+    // a run's last item may be shorter than L (its nz need not be a multiple)
+    const int jend = min(it.y + L, r.nz);
+    pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, it.y, jend, active, ring_a[team], ring_b[team],
                                      yring + (PAIR == 1 ? team * NW * kYRing : 0));
 }
 

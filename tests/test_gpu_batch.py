@@ -41,14 +41,20 @@ def variant(request, monkeypatch):
     return int(request.param)
 
 
+@pytest.mark.parametrize("chunk", ["", "48"])
 @pytest.mark.parametrize("nsteps", [20, 37])
-def test_batch_runs_bit_identical_to_solo_runs(gpu, variant, nsteps):
+def test_batch_runs_bit_identical_to_solo_runs(gpu, variant, nsteps, chunk, monkeypatch):
     # 20 = one 16-step graph replay + 4 eager steps; 37 (odd) leaves the
-    # two-stage batches' states in their second buffers
+    # two-stage batches' states in their second buffers.  Chunks of 48 rows
+    # divide none of the runs' nz but 192: each run's last item is shorter.
+    if chunk:
+        monkeypatch.setenv("COSENU_BATCH_ZCHUNK", chunk)
     runs, ics = _runs()
     with C.Batch(runs) as b:
         info = b.info()
         assert info["nruns"] == len(runs) and info["ngroups"] == 4
+        if chunk:
+            assert info["z_chunk"] == int(chunk)
         assert info["fast_variant"] == variant
         assert info["cells"] == sum(p.nz * p.nvz for p in runs)
         for r, (G, bG, s0) in enumerate(ics):

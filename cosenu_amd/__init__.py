@@ -112,6 +112,8 @@ def lib() -> ctypes.CDLL:
     L.cosenu_batch_create.argtypes = [ctypes.POINTER(_Params), ctypes.c_int, ctypes.POINTER(vp)]
     L.cosenu_batch_destroy.argtypes = [vp]
     L.cosenu_batch_get_info.argtypes = [vp, ctypes.POINTER(_BatchInfo)]
+    L.cosenu_batch_autotune.argtypes = [ctypes.POINTER(_Params), ctypes.c_int, ip,
+                                        ctypes.POINTER(ctypes.c_double)]
     L.cosenu_batch_set_profile.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_ssize_t, ctypes.c_int]
     L.cosenu_batch_upload_planes.argtypes = [vp, ctypes.c_int, ctypes.c_int, _planes_t,
                                              ctypes.c_ssize_t, ctypes.c_int]
@@ -324,6 +326,17 @@ def autotune(p: RunParams):
     _check(lib().cosenu_autotune(ctypes.byref(p.c()), ctypes.byref(v), ctypes.byref(L),
                                  ctypes.byref(ms)), "cosenu_autotune")
     return v.value, L.value, ms.value
+
+
+def batch_autotune(runs: list):
+    """Time the work-item chunk lengths of a batch of these runs (on a
+    scratch batch); the winner is used by batches created afterwards.
+    Returns (z_chunk, ms per batch step)."""
+    arr = (_Params * len(runs))(*[r.c() for r in runs])
+    L, ms = ctypes.c_int(), ctypes.c_double()
+    _check(lib().cosenu_batch_autotune(arr, len(runs), ctypes.byref(L), ctypes.byref(ms)),
+           "cosenu_batch_autotune")
+    return L.value, ms.value
 
 
 class Batch:

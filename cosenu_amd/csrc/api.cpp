@@ -984,6 +984,10 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = fail(COSENU_EHIP, "cosenu_autotune: hipEventCreate failed");
+    // timed on a pseudo-random state and profile, not on zeros
+    if (!rc && (launch_fill_pattern(c->y, (long)c->nz * 8, c->nvz, c->VP, c->stream) != hipSuccess ||
+                launch_fill_pattern(c->prof, (long)c->nz * 2, c->nvz, c->VP, c->stream) != hipSuccess))
+        rc = fail(COSENU_EHIP, "cosenu_autotune: fill failed");
     // chunk lengths, then (variants 2-4) contiguous ranges with 1 or 2 times
     // the resident teams, encoded as chunk = -1, -2
     const int cands[] = {1, 2, 4, 8, 16, 32, 64, 128, 256, -1, -2};
@@ -1135,7 +1139,8 @@ int batch_enqueue_step(cosenu_batch *b, hipStream_t s)
     return 0;
 }
 
-// Largest tuned (or default) chunk that divides every nz of the group.
+// Chunk length of a group's work items: the tuned one (entry nz = -1 of the
+// tune table, or COSENU_BATCH_ZCHUNK), else one item per resident team.
 int group_chunk(const cosenu_batch *b, const Group &g)
 {
     int maxnz = 1;
@@ -1164,6 +1169,34 @@ int group_chunk(const cosenu_batch *b, const Group &g)
     // chunks need not divide a run's nz: a run's last work item takes the
     // remaining rows (the kernels take ranges shorter than the chunk)
     return std::max(1, std::min(want, maxnz));
+}
+
+// (Re)build a group's work items, (run, first z cell), for chunk length L
+// (at most the group's largest nz).
+hipError_t set_group_chunk(cosenu_batch *b, Group &g, int L)
+{
+    int maxnz = 1;
+    for (int r : g.runs)
+        maxnz = std::max(maxnz, b->P[r].nz);
+    L = std::max(1, std::min(L, maxnz));
+    std::vector<int2> items;
+    for (size_t k = 0; k < g.runs.size(); k++)
+        for (int j0 = 0; j0 < b->P[g.runs[k]].nz; j0 += L)
+            items.push_back(make_int2((int)k, j0));
+    if (g.d_items && (int)items.size() > g.nitems) {
+        (void)hipFree(g.d_items);
+        g.d_items = nullptr;
+    }
+    hipError_t e = hipSuccess;
+    if (!g.d_items)
+        e = hipMalloc((void **)&g.d_items, items.size() * sizeof(int2));
+    if (e == hipSuccess)
+        e = hipMemcpy(g.d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        g.L = L;
+        g.nitems = (int)items.size();
+    }
+    return e;
 }
 
 } // namespace
@@ -1266,7 +1299,6 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
         g.uniform = true;
         std::vector<BatchRun> hr;
         std::vector<double> vz_all, hm_all;
-        std::vector<int2> items;
         size_t vo = 0, ho = 0;
         for (size_t k = 0; k < g.runs.size(); k++) {
             const int r = g.runs[k];
@@ -1287,8 +1319,6 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
             br.dz = d.dz; br.dv = d.dv; br.ct = d.ct; br.st = d.st;
             br.pmo = p.pmo; br.mu = p.mu; br.ko = p.ko;
             hr.push_back(br);
-            for (int j0 = 0; j0 < p.nz; j0 += g.L)
-                items.push_back(make_int2((int)k, j0));
             b->run_y[r] = br.y;
             b->run_sp[r] = br.acc;
             View &v = b->views[r];
@@ -1299,7 +1329,6 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
             vo += g.VP;
             ho += p.nz;
         }
-        g.nitems = (int)items.size();
         std::vector<BatchRun> hs = hr;
         for (BatchRun &br : hs)
             std::swap(br.y, br.acc);
@@ -1307,11 +1336,10 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
         if (e == hipSuccess) e = hipMalloc((void **)&g.d_runs_sw, hs.size() * sizeof(BatchRun));
         if (e == hipSuccess)
             e = hipMemcpy(g.d_runs_sw, hs.data(), hs.size() * sizeof(BatchRun), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMalloc((void **)&g.d_items, items.size() * sizeof(int2));
         if (e == hipSuccess)
             e = hipMemcpy(g.d_runs, hr.data(), hr.size() * sizeof(BatchRun), hipMemcpyHostToDevice);
         if (e == hipSuccess)
-            e = hipMemcpy(g.d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice);
+            e = set_group_chunk(b, g, g.L);
         if (e == hipSuccess)
             e = hipMemcpy(g.vz, vz_all.data(), vz_all.size() * sizeof(double), hipMemcpyHostToDevice);
         if (e == hipSuccess)
@@ -1453,6 +1481,64 @@ int cosenu_batch_get_info(cosenu_batch *b, cosenu_batch_info *info)
         info->work_items += g.nitems;
     info->z_chunk = b->groups.empty() ? 0 : b->groups[0].L;
     info->fast_variant = b->pairs ? 4 : 2;
+    return 0;
+}
+
+int cosenu_batch_autotune(const cosenu_params *runs, int nruns, int *z_chunk, double *ms_best)
+{
+    cosenu_batch *b = nullptr;
+    RC_TRY(cosenu_batch_create(runs, nruns, &b));
+    int rc = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = fail(COSENU_EHIP, "cosenu_batch_autotune: hipEventCreate failed");
+    // timed on pseudo-random states and profiles, not on zeros
+    for (Group &g : b->groups)
+        if (!rc && (launch_fill_pattern(g.y, (long)(g.n_state / g.VP), g.VP, g.VP, b->stream) != hipSuccess ||
+                    launch_fill_pattern(g.prof, (long)(g.n_prof / g.VP), g.VP, g.VP, b->stream) != hipSuccess))
+            rc = fail(COSENU_EHIP, "cosenu_batch_autotune: fill failed");
+    double best = 1e300;
+    int bl = 0;
+    // chunk lengths of the work items; whole batch steps (zero states) timed
+    const int cands[] = {32, 64, 128, 256, 512};
+    for (int L : cands) {
+        for (Group &g : b->groups)
+            if (!rc && set_group_chunk(b, g, L) != hipSuccess)
+                rc = fail(COSENU_EHIP, "cosenu_batch_autotune: work items");
+        const int reps = 3;
+        for (int k = 0; k < 1 + reps && !rc; k++) {
+            if (k == 1 && hipEventRecord(e0, b->stream) != hipSuccess)
+                rc = fail(COSENU_EHIP, "cosenu_batch_autotune: event record");
+            if (!rc)
+                rc = batch_enqueue_step(b, b->stream);
+        }
+        if (rc)
+            break;
+        float ms = 0;
+        if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(COSENU_EHIP, "cosenu_batch_autotune: timing failed");
+            break;
+        }
+        if (ms / reps < best) {
+            best = ms / reps;
+            bl = L;
+        }
+    }
+    if (!rc)
+        for (const Group &g : b->groups) // batches created afterwards use it
+            tune_store(g.VP, g.scheme, -1, {b->pairs ? 4 : 2, bl, best});
+    if (e0)
+        (void)hipEventDestroy(e0);
+    if (e1)
+        (void)hipEventDestroy(e1);
+    cosenu_batch_destroy(b);
+    if (rc)
+        return rc;
+    if (z_chunk)
+        *z_chunk = bl;
+    if (ms_best)
+        *ms_best = best;
     return 0;
 }
 

@@ -254,7 +254,29 @@ __global__ __launch_bounds__(256) void k_profile_v(double *dst, const double *G,
     }
 }
 
+// rows of VP doubles: v < nvz gets a pseudo-random value in [0.5, 1) (the
+// SplitMix64 finaliser of the element index), padding bins zero.  Timing
+// data for the autotuners: all-zero operands draw less power, so the chip
+// clocks higher than on a real state and launch shapes rank differently.
+__global__ __launch_bounds__(256) void k_fill_pattern(double *dst, long nrows, int nvz, int VP)
+{
+    const long n = nrows * VP;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        unsigned long long x = (unsigned long long)i * 0x9E3779B97F4A7C15ull;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        x ^= x >> 31;
+        dst[i] = (int)(i % VP) < nvz ? 0.5 + 0x1p-54 * (double)(x >> 11) : 0.0;
+    }
+}
+
 } // namespace
+
+hipError_t launch_fill_pattern(double *dst, long nrows, int nvz, int VP, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_pattern, dim3(4096), dim3(256), 0, s, dst, nrows, nvz, VP);
+    return hipGetLastError();
+}
 
 hipError_t launch_separable(double *dst, const double *a, const double *b, int nz, int nvz, int VP,
                             hipStream_t s)

@@ -187,5 +187,7 @@ hipError_t launch_separable(double *dst, const double *a, const double *b, int n
                             hipStream_t s);
 hipError_t launch_profile_v(double *dst, const double *G, const double *bG, int nz, int nvz, int VP,
                             hipStream_t s);
+// nrows rows of VP doubles: bins v < nvz pseudo-random in [0.5, 1), padding 0
+hipError_t launch_fill_pattern(double *dst, long nrows, int nvz, int VP, hipStream_t s);
 
 } // namespace cosenu

@@ -7,11 +7,13 @@ seeds with per-run SplitMix64 seeds; z in [-600, 600], CFL 0.4, mu = 1,
 omega = 0, FV.  (The z range and step count are not fixed by the workload's
 definition; these are the recorded choices.)
 
-    python -m cosenu_amd.sweep [--runs 256] [--steps 100] [--check 8]
+    python -m cosenu_amd.sweep [--runs 256] [--steps 100] [--check 8] [--autotune]
 
 prints one JSON line: aggregate cell-updates/s of the batch and the
 agreement of `--check` randomly chosen runs with the same runs advanced
-alone (bit-identical expected).
+alone (bit-identical expected).  `--autotune` first times the batch's
+work-item chunk lengths on a scratch batch (cosenu_batch_autotune) and the
+batch uses the fastest.
 """
 from __future__ import annotations
 
@@ -23,7 +25,7 @@ import time
 
 import numpy as np
 
-from . import COLL, FAST, FV, STATE, STATE0, VAC, Batch, RunParams, Simulation
+from . import COLL, FAST, FV, STATE, STATE0, VAC, Batch, RunParams, Simulation, batch_autotune
 from .initial import seeded_ic
 
 
@@ -47,9 +49,15 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--check", type=int, default=8)
+    ap.add_argument("--autotune", action="store_true")
     args = ap.parse_args(argv)
 
     runs, ics = sweep_runs(args.runs)
+    tuned = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        L, ms = batch_autotune(runs)
+        tuned = {"z_chunk": L, "ms_per_step": ms, "seconds": time.perf_counter() - t0}
     t0 = time.perf_counter()
     B = Batch(runs)
     for r, (p, (sn, sb, al, sd)) in enumerate(zip(runs, ics)):
@@ -87,7 +95,8 @@ def main(argv=None):
            "groups": info["ngroups"], "z_chunk": info["z_chunk"], "steps": args.steps,
            "ms_per_step": 1e3 * el / args.steps, "setup_s": t_setup,
            "device_GB": info["device_bytes"] / 1e9,
-           "checked_runs": pick, "bit_identical_to_solo": f"{same}/{len(pick)}"}
+           "checked_runs": pick, "bit_identical_to_solo": f"{same}/{len(pick)}",
+           "autotune": tuned}
     print(json.dumps(out), flush=True)
     return 0 if same == len(pick) else 1
 

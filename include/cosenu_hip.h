@@ -218,6 +218,16 @@ typedef struct cosenu_batch_info {
 int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out);
 int cosenu_batch_destroy(cosenu_batch *b);
 int cosenu_batch_get_info(cosenu_batch *b, cosenu_batch_info *info);
+
+/* Tile scheduler tuning for batches (no counterpart in the reference, whose
+ * sweeps are one process per job, manage.py:361-428).  Builds a scratch
+ * batch of the given runs, times whole FAST steps for work-item chunks of
+ * 32..512 z cells (at most a group's largest nz; a run's last item is
+ * shorter) and records the fastest for every (nvz rounded to 64, scheme)
+ * group of the runs in the tune table (entry nz = -1; COSENU_TUNE_FILE as for
+ * cosenu_autotune).  Batches created afterwards use it.  Needs the device
+ * memory of the batch; *ms is per batch step. */
+int cosenu_batch_autotune(const cosenu_params *runs, int nruns, int *z_chunk, double *ms);
 int cosenu_batch_set_profile(cosenu_batch *b, int run, const double *G, const double *bG,
                              ptrdiff_t row_stride, int row_offset);
 int cosenu_batch_upload_planes(cosenu_batch *b, int run, int which, const double *const planes[8],

@@ -100,6 +100,33 @@ def test_batch_rejects_bad_runs(gpu):
         C.Batch([])
 
 
+def test_batch_autotune_records_and_applies(gpu, tmp_path, monkeypatch):
+    table = tmp_path / "tune.txt"
+    monkeypatch.setenv("COSENU_TUNE_FILE", str(table))
+    monkeypatch.delenv("COSENU_BATCH_ZCHUNK", raising=False)
+    monkeypatch.delenv("COSENU_BATCH_VARIANT", raising=False)
+    runs, ics = _runs()
+    L, ms = C.batch_autotune(runs)
+    assert L in (32, 64, 128, 256, 512) and ms > 0
+    rows = [l.split() for l in table.read_text().splitlines() if not l.startswith("#")]
+    groups = {(r[0], r[1]) for r in rows if r[2] == "-1"}
+    assert groups == {("64", "0"), ("128", "0"), ("64", "1"), ("256", "0")}
+    assert all(r[4] == str(L) for r in rows if r[2] == "-1")
+    with C.Batch(runs) as b:
+        # the first group (nvz 64, FV: nz 256 and 128) caps the chunk at its largest nz
+        assert b.info()["z_chunk"] == min(L, 256)
+        for r, (G, bG, s0) in enumerate(ics):
+            b.upload(r, s0)
+        b.step(6)
+        out = b.download(0)
+    p = runs[0]
+    G, bG, s0 = ics[0]
+    with C.Simulation(p) as sim:
+        sim.upload(s0)
+        sim.step(6)
+        assert np.array_equal(out, sim.download())
+
+
 def test_autotune_records_and_applies(gpu, tmp_path, monkeypatch):
     table = tmp_path / "tune.txt"
     monkeypatch.setenv("COSENU_TUNE_FILE", str(table))
@@ -110,7 +137,7 @@ def test_autotune_records_and_applies(gpu, tmp_path, monkeypatch):
     assert v in (1, 2, 3, 4) and ms > 0
     assert (L > 0 and 4096 % L == 0) or (v >= 2 and L in (-1, -2))
     rows = [l.split() for l in table.read_text().splitlines() if not l.startswith("#")]
-    assert ["128", "0", "4096", str(v), str(L)] == rows[-1][:5]
+    assert [["128", "0", "4096", str(v), str(L)]] == [r[:5] for r in rows if r[2] == "4096"]
     with C.Simulation(p) as sim:
         info = sim.info()
     assert info["fast_variant"] == v

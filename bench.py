@@ -122,16 +122,23 @@ def main():
 
     sim.step(args.warmup)
     sim.sync()
-    sim.set_timing(True)   # HIP events around every stage launch on the library's stream
-    sim.sync()
     t0 = time.perf_counter()
-    sim.step(args.steps)
+    sim.step(args.steps)   # graph replay, as a caller steps
     sim.sync()
     el = time.perf_counter() - t0
-    ms, cnt = sim.timing()
-    sim.set_timing(False)
     value = cells * args.steps / el
     ms_step = 1e3 * el / args.steps
+    # Per-launch durations: the same number of steps again, with HIP events
+    # recorded around every launch on the library's stream.  (The events
+    # replace graph replay and add a gap per launch: at the 1024 x 64 size,
+    # where a step is ~30 us, they would lengthen it by a third, so the
+    # throughput above is timed without them.)
+    sim.set_timing(True)
+    sim.sync()
+    sim.step(args.steps)
+    sim.sync()
+    ms, cnt = sim.timing()
+    sim.set_timing(False)
 
     # Per-kernel times from HIP events on the library's stream.  Stages 2 and
     # 3 share one kernel; the dominant kernel is the one with the largest

@@ -121,6 +121,7 @@ def test_batch_autotune_records_and_applies(gpu, tmp_path, monkeypatch):
         out = b.download(0)
     p = runs[0]
     G, bG, s0 = ics[0]
+    monkeypatch.setenv("COSENU_FAST_VARIANT", "4")  # the batch's kernel
     with C.Simulation(p) as sim:
         sim.upload(s0)
         sim.step(6)

@@ -2,7 +2,8 @@
 32 velocity bins in the upwind direction and reads the per-z moments of its
 input from the launch that wrote it.
 
-It is selected with COSENU_FAST_VARIANT=3 (or by the autotuner) where it
+It is the default for small problems (at most 2 wavefronts per SIMD in all)
+and is selected with COSENU_FAST_VARIANT=3 (or by the autotuner) where it
 applies (nvz a multiple of 64, periodic boundaries, every group of 32 bins of
 one sign of v); elsewhere the split-lane kernel (variant 2) runs.  Both
 evaluate the same mathematics with different summation orders, so they agree
@@ -117,6 +118,25 @@ def test_mixed_sign_groups_fall_back_to_split_lane(gpu):
     ref = O.step(po, s, 6)
     with C.Simulation(pc) as sim:
         assert sim.info()["fast_variant"] == 2
+        sim.upload(s)
+        sim.step(6)
+        out = sim.download()
+    assert _rel(out, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("scheme", [O.FV, O.FD])
+def test_small_problem_default_is_barrier_free(gpu, scheme, monkeypatch):
+    """Without a tuned entry or override, a problem of at most 2 wavefronts
+    per SIMD in all (nz = 1024, nvz = 64: the CPU-sized case's grid) runs the
+    barrier-free kernel in chunks of 2 rows; its steps match the oracle."""
+    monkeypatch.delenv("COSENU_FAST_VARIANT", raising=False)
+    monkeypatch.delenv("COSENU_ZCHUNK", raising=False)
+    po, pc = _pair(scheme, 1024, 64)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=23, amp=1e-3)
+    ref = O.step(po, s, 6)
+    with C.Simulation(pc) as sim:
+        info = sim.info()
+        assert info["fast_variant"] == 3 and info["z_chunk"] == 2 and info["range_teams"] == 0
         sim.upload(s)
         sim.step(6)
         out = sim.download()

@@ -8,6 +8,8 @@ agree normwise: ||gpu - ref||_inf / ||ref||_inf <= 1e-12 for one RHS of a
 smooth state (1e-9 for a state with 1e-4 noise, where the reference's own
 arithmetic is ill-conditioned) and <= 1e-10 after RK4 steps.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -294,3 +296,27 @@ def test_fast_open_boundary_and_ko5(gpu, scheme, bc, flags, nvz):
         sim.step(30)
         out = sim.download()
     assert _rel(out, O.step(po, s, 30)) <= 1e-10
+
+
+@pytest.mark.parametrize("variant", ["2", "4"])
+def test_fast_flavor_growth_rate_matches_linear_theory(gpu, monkeypatch, variant):
+    """The FAST kernels reproduce the linear growth and turning rate of the
+    homogeneous fast-flavour instability (the same check as the oracle's,
+    tests/test_oracle_math.py::test_fast_flavor_growth_rate_matches_linear_theory)."""
+    from test_oracle_math import ffi_growth_check
+    monkeypatch.setenv("COSENU_FAST_VARIANT", variant)
+
+    def gpu_step(p, s, n):
+        pc = C.RunParams(nz=p.nz, nvz=p.nvz, z0=p.z0, z1=p.z1, CFL=p.CFL, flags=p.flags,
+                         mu=p.mu, mode=C.FAST)
+        with C.Simulation(pc) as sim:
+            assert sim.info()["fast_variant"] == int(variant)
+            sim.upload(s)
+            sim.step(n)
+            return sim.download()
+
+    rate, lam, resid, t = ffi_growth_check(gpu_step)
+    assert abs(rate.real - lam.real) <= 1e-5 * lam.real
+    turn = (lam.imag - rate.imag) * t / (2 * math.pi)
+    assert abs(turn - round(turn)) < 1e-6
+    assert resid < 1e-12

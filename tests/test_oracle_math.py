@@ -194,3 +194,76 @@ def test_reference_weno_conditioning(oracle_lib):
         smooth.append(abs(O.weno7(u)[0] - float(weno_ld(u))))
     assert 1e-12 < max(noisy) < 1e-9     # ill-conditioned regime
     assert max(smooth) < 1e-14           # benign regime
+
+
+def _ffi_mode(nvz, sig_nu, sig_nubar, alpha, mu=1.0):
+    """Linearised collective term about rho = diag(G, 0), rhobar = diag(bG, 0)
+    (rhs_fv.hpp:188-233 to first order in the coherences), as a real matrix
+    on x = (ex_re, ex_im, bex_re, bex_im) of the nvz bins:
+      S = W (ex_im + bex_im), R = W (ex_re - bex_re), D = W (G - bG),
+      W_ik = mu (1 - v_i v_k),
+      d ex_re = dv (-G S + D ex_im),   d ex_im = dv (G R - D ex_re),
+      d bex_re = dv (-bG S - D bex_im), d bex_im = dv (-bG R + D bex_re).
+    Returns (G, bG, its fastest-growing eigenvalue and eigenvector)."""
+    from cosenu_amd.initial import g_profile
+    dv = 2.0 / nvz
+    v = -1.0 + (np.arange(nvz) + 0.5) * dv
+    G = np.array([g_profile(x, 1.0, sig_nu) for x in v])
+    bG = alpha * np.array([g_profile(x, 1.0, sig_nubar) for x in v])
+    W = mu * (1.0 - np.outer(v, v))
+    D = np.diag(W @ (G - bG))
+    n = nvz
+    A = np.zeros((4 * n, 4 * n))
+    er, ei, br, bi = (slice(k * n, (k + 1) * n) for k in range(4))
+    GW, bGW = np.diag(G) @ W, np.diag(bG) @ W
+    A[er, ei] = -GW + D
+    A[er, bi] = -GW
+    A[ei, er] = GW - D
+    A[ei, br] = -GW
+    A[br, ei] = -bGW
+    A[br, bi] = -bGW - D
+    A[bi, er] = -bGW
+    A[bi, br] = bGW + D
+    lam, vec = np.linalg.eig(dv * A)
+    k = int(np.argmax(lam.real))
+    return G, bG, lam[k], vec[:, k]
+
+
+def ffi_growth_check(step):
+    """Homogeneous fast-flavour instability: a z-uniform state started on
+    the fastest-growing eigenvector of the linearised operator must grow at
+    Re(lambda) and turn at Im(lambda).  `step(params, state, n)` advances a
+    state [8, nvz, nz]; returns (measured rate, lambda, eigenspace residual)."""
+    nvz, nz = 32, 16
+    G, bG, lam, w = _ffi_mode(nvz, 0.6, 0.5, 0.9)
+    assert lam.real > 5e-3  # an ELN crossing: unstable
+    p = O.RunParams(nz=nz, nvz=nvz, z0=0.0, z1=20.0, CFL=0.4, flags=O.COLL | O.ADVEC_OFF, mu=1.0)
+    dt = p.CFL * (p.z1 - p.z0) / nz  # 0.5: dt |lambda| = 0.05
+    x0 = 1e-10 * w.real / np.linalg.norm(w.real)
+    s = np.zeros((8, nvz, nz))
+    s[0], s[4] = G[:, None], bG[:, None]
+    for k, f in enumerate((2, 3, 6, 7)):
+        s[f] = x0[k * nvz:(k + 1) * nvz, None]
+    B = np.stack([w.real, -w.imag], axis=1)  # x = Re(c w): columns for (Re c, Im c)
+
+    def amplitude(st):
+        assert np.ptp(st[2:4], axis=2).max() == 0.0  # still z-uniform
+        x = np.concatenate([st[f][:, 0] for f in (2, 3, 6, 7)])
+        c = np.linalg.lstsq(B, x, rcond=None)[0]
+        return complex(c[0], c[1]), np.linalg.norm(B @ c - x) / np.linalg.norm(x)
+
+    n = 200
+    s1 = step(p, s, n)
+    s2 = step(p, s1, n)
+    (c1, r1), (c2, r2) = amplitude(s1), amplitude(s2)
+    t = n * dt
+    rate = complex(math.log(abs(c2) / abs(c1)) / t, np.angle(c2 / c1) / t)
+    return rate, lam, max(r1, r2), t
+
+
+def test_fast_flavor_growth_rate_matches_linear_theory(oracle_lib):
+    rate, lam, resid, t = ffi_growth_check(O.step)
+    assert abs(rate.real - lam.real) <= 1e-5 * lam.real
+    turn = (lam.imag * t - rate.imag * t) / (2 * math.pi)  # the phase, modulo 2 pi
+    assert abs(turn - round(turn)) < 1e-6
+    assert resid < 1e-12

@@ -320,3 +320,23 @@ def test_fast_flavor_growth_rate_matches_linear_theory(gpu, monkeypatch, variant
     turn = (lam.imag - rate.imag) * t / (2 * math.pi)
     assert abs(turn - round(turn)) < 1e-6
     assert resid < 1e-12
+
+
+@pytest.mark.parametrize("variant", ["2", "4"])
+def test_polarisation_length_conserved_up_to_rk4_truncation(gpu, monkeypatch, variant):
+    """As the oracle's test (tests/test_oracle_math.py): |P| per cell changes
+    only by the RK4 truncation error under the non-advective terms."""
+    from test_oracle_math import plength_drift
+    monkeypatch.setenv("COSENU_FAST_VARIANT", variant)
+
+    def gpu_step(p, s, n):
+        pc = C.RunParams(nz=p.nz, nvz=p.nvz, z0=p.z0, z1=p.z1, CFL=p.CFL, flags=p.flags,
+                         omega=p.omega, theta=p.theta, mu=p.mu, mode=C.FAST)
+        with C.Simulation(pc) as sim:
+            sim.upload(s)
+            sim.step(n)
+            return sim.download()
+
+    e1, e2 = plength_drift(gpu_step, 0.02), plength_drift(gpu_step, 0.01)
+    assert e2 < 1e-10
+    assert 12.0 < e1 / e2 < 20.0, (e1, e2)

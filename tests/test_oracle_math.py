@@ -267,3 +267,33 @@ def test_fast_flavor_growth_rate_matches_linear_theory(oracle_lib):
     turn = (lam.imag * t - rate.imag * t) / (2 * math.pi)  # the phase, modulo 2 pi
     assert abs(turn - round(turn)) < 1e-6
     assert resid < 1e-12
+
+
+def plength_drift(step, dt):
+    """Largest relative change of the polarisation-vector lengths
+    |P| = sqrt((ee - xx)^2 + 4 |ex|^2) (and |Pbar|) over the cells after a
+    time 2 of vacuum + collective evolution without advection: the
+    non-advective terms rotate each cell's P (the commutator form of
+    rhs_fv.hpp:164-233), so |P| changes only by the RK4 truncation error."""
+    rng = np.random.default_rng(5)
+    nvz, nz = 16, 16
+    s = np.empty((8, nvz, nz))
+    s[0], s[1], s[4], s[5] = rng.uniform(0.1, 1.0, (4, nvz, nz))
+    s[2], s[3], s[6], s[7] = rng.uniform(-0.1, 0.1, (4, nvz, nz))
+    L = dt * nz / 0.4
+    p = O.RunParams(nz=nz, nvz=nvz, z0=0.0, z1=L, CFL=0.4, flags=O.VAC | O.COLL | O.ADVEC_OFF,
+                    omega=0.5, theta=20.0, mu=1.0)
+    out = step(p, s, int(round(2.0 / dt)))
+
+    def lengths(x):
+        return (np.sqrt((x[0] - x[1]) ** 2 + 4 * (x[2] ** 2 + x[3] ** 2)),
+                np.sqrt((x[4] - x[5]) ** 2 + 4 * (x[6] ** 2 + x[7] ** 2)))
+
+    (a0, b0), (a1, b1) = lengths(s), lengths(out)
+    return max(np.max(np.abs(a1 - a0) / a0), np.max(np.abs(b1 - b0) / b0))
+
+
+def test_polarisation_length_conserved_up_to_rk4_truncation(oracle_lib):
+    e1, e2 = plength_drift(O.step, 0.02), plength_drift(O.step, 0.01)
+    assert e2 < 1e-10
+    assert 12.0 < e1 / e2 < 20.0, (e1, e2)  # RK4: dt^4

@@ -46,8 +46,11 @@ BYTES_PER_CELL_STEP_DEF = sum(ALG_STAGE_BYTES)
 #      stages 1+2 read y, write Y3 and Q = (Y2 - y) + 2 (Y3 - y) (64+128);
 #      stages 3+4 read Y3, y, Q, write the new state (192+64); the timing
 #      slots are the two launches
+#   5 (FD: four stages per launch; stage outputs stay in registers, the
+#      increment sum in LDS): read y, write the new state (64+64), plus the
+#      halo rows each team reads again (12 rows per team range)
 STAGE_BYTES = {1: (192, 320, 320, 256), 2: (128, 192, 192, 320), 3: (132, 196, 196, 324),
-               4: (192, 256, 0, 0)}
+               4: (192, 256, 0, 0), 5: (128, 0, 0, 0)}
 
 
 def cpu_baseline(seconds_target: float = 12.0):
@@ -148,7 +151,9 @@ def main():
     ab = ALG_STAGE_BYTES
     avg = [m / max(1, c) for m, c in zip(ms, cnt)]
     # name: (avg ms per launch, algorithmic bytes/cell, moved bytes/cell, kernel, launches per step)
-    if variant == 4:
+    if variant == 5:
+        kernels = {"stages1-4": (avg[0], sum(ab), sb[0], sim.stage_kernel_name(0), 1)}
+    elif variant == 4:
         kernels = {"stages1+2": (avg[0], ab[0] + ab[1], sb[0], sim.stage_kernel_name(0), 1),
                    "stages3+4": (avg[1], ab[2] + ab[3], sb[1], sim.stage_kernel_name(1), 1)}
     else:

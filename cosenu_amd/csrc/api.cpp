@@ -308,6 +308,7 @@ int range_teams(int nz, int VP, int scheme, int bc, int variant, int mult)
 {
     long t = variant == 3   ? (long)stream_resident_waves(scheme) / (VP / 32)
              : variant == 4 ? (long)pair_resident_teams(VP, scheme)
+             : variant == 5 ? (long)quad_resident_teams(VP)
                             : (long)fast_resident_teams(VP, scheme, bc);
     t *= mult;
     if (variant == 4 && mult == 1 && t >= 2 && (nz + t - 1) / t < (scheme == 0 ? 8 : 3))
@@ -445,6 +446,9 @@ hipError_t launch_stage(const cosenu_ctx *c, int stage, hipStream_t s)
 }
 
 bool two_stage_launches(const cosenu_ctx *c) { return c->p.mode == COSENU_FAST && c->variant == 4; }
+bool four_stage_launch(const cosenu_ctx *c) { return c->p.mode == COSENU_FAST && c->variant == 5; }
+// the kernels that write the new state to the spare buffer
+bool swaps_state(const cosenu_ctx *c) { return two_stage_launches(c) || four_stage_launch(c); }
 
 // Stages 1+2 (pair 0) or 3+4 (pair 1) in one launch; the new state of pair 1
 // goes to the spare state buffer (acc), which becomes y after the step.
@@ -462,19 +466,28 @@ void swap_state(cosenu_ctx *c)
     c->view.y = c->y;
 }
 
+// All four stages in one launch; the new state goes to acc.
+hipError_t launch_quad(const cosenu_ctx *c, hipStream_t s)
+{
+    const StageArgs a = stage_args(c, 1);
+    const double dt = c->d.dt;
+    const double ca[4] = {0.5 * dt, 0.5 * dt, dt, 1.0 / 6.0 * dt};
+    return launch_stage_quad(a, ca, c->acc, c->z_chunk, c->nteams, c->uniform, s);
+}
+
 // The launches of one RK4 step; with timing, events around each launch
 // (timing slot k = launch k of the step).
 int enqueue_step(cosenu_ctx *c, hipStream_t s)
 {
-    const bool pairs = two_stage_launches(c);
-    for (int k = 0; k < (pairs ? 2 : 4); k++) {
+    const bool pairs = two_stage_launches(c), quad = four_stage_launch(c);
+    for (int k = 0; k < (quad ? 1 : pairs ? 2 : 4); k++) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (c->timing) {
             HIP_TRY(hipEventCreate(&e0));
             HIP_TRY(hipEventCreate(&e1));
             HIP_TRY(hipEventRecord(e0, s));
         }
-        HIP_TRY(pairs ? launch_pair(c, k, s) : launch_stage(c, k + 1, s));
+        HIP_TRY(quad ? launch_quad(c, s) : pairs ? launch_pair(c, k, s) : launch_stage(c, k + 1, s));
         if (c->timing) {
             HIP_TRY(hipEventRecord(e1, s));
             c->ev.push_back(e0);
@@ -482,7 +495,7 @@ int enqueue_step(cosenu_ctx *c, hipStream_t s)
             c->ev_stage.push_back(k);
         }
     }
-    if (pairs)
+    if (pairs || quad)
         swap_state(c);
     return 0;
 }
@@ -572,7 +585,7 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             chunk = -1;
         }
         int ev = env_int("COSENU_FAST_VARIANT", 0), ez = env_int("COSENU_ZCHUNK", 0);
-        if (ev >= 1 && ev <= 4 && ev != c->variant) {
+        if (ev >= 1 && ev <= 5 && ev != c->variant) {
             c->variant = ev;
             chunk = -1; // that variant's default launch: contiguous ranges
         }
@@ -580,6 +593,8 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             chunk = ez;
         if (c->variant == 3 && !c->stream_ok)
             c->variant = 2; // shapes the barrier-free kernel does not take
+        if (c->variant == 5 && (!quad_supported(c->VP, p->bc, p->scheme, p->flags) || c->nz < 16))
+            c->variant = 4; // FV, 5th-order dissipation, open boundary, nvz > 256
         if (c->variant == 4 && (!pair_supported(c->VP, p->bc) || c->nz < 16))
             c->variant = 2;
         if (p->bc == COSENU_OPEN)
@@ -803,10 +818,10 @@ int cosenu_step(cosenu_ctx *c, int nsteps)
         // the state may have been replaced since the last step: its moments
         // are rebuilt here, after that every stage 4 leaves them current
         HIP_TRY(launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->VP, c->stream));
-    // two-stage launches alternate the state buffers every step: a graph of
-    // an even number of steps ends on the buffer it started from
+    // two- and four-stage launches alternate the state buffers every step: a
+    // graph of an even number of steps ends on the buffer it started from
     int gsteps = env_int("COSENU_GRAPH_STEPS", 16);
-    if (two_stage_launches(c))
+    if (swaps_state(c))
         gsteps &= ~1;
     int done = 0;
     if (!c->timing && gsteps > 0 && nsteps >= gsteps) {
@@ -858,8 +873,8 @@ int cosenu_rhs(cosenu_ctx *c, double *const out[8], ptrdiff_t stride, int off)
         a.yout = c->B;
         a.mom_out = nullptr;
         HIP_TRY(launch_stage_stream(a, 0, c->z_chunk, c->nteams, c->stream));
-    } else if (c->p.mode == COSENU_FAST) { // variant 4 evaluates single RHSs with variant 2
-        HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->variant == 4 ? 2 : c->variant, c->nteams,
+    } else if (c->p.mode == COSENU_FAST) { // variants 4, 5 evaluate single RHSs with variant 2
+        HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->variant >= 4 ? 2 : c->variant, c->nteams,
                                 c->stream));
     } else {
         HIP_TRY(launch_rhs_strict(r, c->stream));
@@ -969,6 +984,8 @@ const char *cosenu_stage_kernel_name(cosenu_ctx *c, int stage)
         return "";
     if (c->p.mode == COSENU_FAST && c->variant == 3)
         return stage_stream_kernel_name(stage);
+    if (four_stage_launch(c))
+        return stage == 0 ? stage_quad_kernel_name() : "";
     if (two_stage_launches(c))
         return stage < 2 ? stage_pair_kernel_name(stage) : "";
     return c->p.mode == COSENU_FAST ? stage_fast_kernel_name(c->VP, stage, c->variant)
@@ -996,14 +1013,15 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     if (!rc && (launch_fill_pattern(c->y, (long)c->nz * 8, c->nvz, c->VP, c->stream) != hipSuccess ||
                 launch_fill_pattern(c->prof, (long)c->nz * 2, c->nvz, c->VP, c->stream) != hipSuccess))
         rc = fail(COSENU_EHIP, "cosenu_autotune: fill failed");
-    // chunk lengths, then (variants 2-4) contiguous ranges with 1 or 2 times
+    // chunk lengths, then (variants 2-5) contiguous ranges with 1 or 2 times
     // the resident teams, encoded as chunk = -1, -2
     const int cands[] = {1, 2, 4, 8, 16, 32, 64, 128, 256, -1, -2};
-    for (int var = q.bc == COSENU_OPEN ? 2 : 1; var <= 4 && !rc; var++)
+    for (int var = q.bc == COSENU_OPEN ? 2 : 1; var <= 5 && !rc; var++)
         for (int L : cands) {
             if (rc || (L > 0 && (L > c->nz || c->nz % L)) || (L < 0 && var == 1) ||
                 (var == 3 && !c->stream_ok) ||
-                (var == 4 && (!pair_supported(c->VP, q.bc) || c->nz < 16)))
+                (var == 4 && (!pair_supported(c->VP, q.bc) || c->nz < 16)) ||
+                (var == 5 && (!quad_supported(c->VP, q.bc, q.scheme, q.flags) || c->nz < 16)))
                 continue;
             c->variant = var;
             if (var == 3 && launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->VP,

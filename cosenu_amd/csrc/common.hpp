@@ -151,6 +151,15 @@ int pair_resident_teams(int VP, int scheme);
 hipError_t launch_stage_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
                                    int scheme, int L, int pair, bool uniform, hipStream_t s);
 const char *stage_pair_kernel_name(int pair);
+
+// FAST mode, FD with 3rd-order dissipation: all four RK4 stages in one
+// launch (variant 5, stage_quad.hip).  `a` = stage 1's arguments (yin = y);
+// ca = dt/2, dt/2, dt, dt/6; the new state goes to yout.
+int quad_supported(int VP, int bc, int scheme, int flags);
+hipError_t launch_stage_quad(const StageArgs &a, const double ca[4], double *yout, int z_chunk,
+                             int nteams, bool uniform, hipStream_t s);
+int quad_resident_teams(int VP);
+const char *stage_quad_kernel_name();
 const char *stage_fast_kernel_name(int VP, int stage, int variant);
 // Batched FAST stage over `nitems` work items (run index, first z cell) of
 // runs with the same VP and scheme; `runs` and `items` are device arrays.

@@ -1,0 +1,372 @@
+// FAST mode, FD scheme: all four RK4 stages in one sweep (variant 5).
+//
+// The finite-difference right-hand side with 3rd-order Kreiss-Oliger
+// dissipation (rhs_fd.hpp:6-148) reads z +- 2 only, so a stage's input
+// window is 5 rows.  The split-lane team of the two-stage kernel (VP/32
+// wavefronts covering all velocity bins of a z range, lanes 0-31 carrying ee,
+// xx, ex_re, ex_im and lanes 32-63 the barred fields of 32 bins) marches
+// along z with four register windows: y, read from HBM, and the outputs of
+// stages 1, 2 and 3, produced in registers.  Stage s runs 2 cells behind
+// stage s - 1 (iteration i: stage 1 at cell i, stage 2 at i - 2, stage 3 at
+// i - 4, stage 4 at i - 6), so every window holds values already computed.
+// Each stage covers the team's cells plus 2 (4 - s) on each side, which the
+// later stages need for their stencils.
+//
+// RK4 (nuosc.hpp:320-348) in the stage-value form of the other FAST kernels:
+//   Y2 = y + (dt/2) K1, Y3 = y + (dt/2) K2, Y4 = y + dt K3,
+//   y' = y + ((Q + (Y4 - y))/3 + (dt/6) K4), Q = (Y2 - y) + 2 (Y3 - y).
+// Q is accumulated per cell in a wavefront-private LDS ring (8 cells); y at
+// the cells of stages 3 and 4 is read again (4 and 6 rows after it left the
+// first window: from L2).  HBM traffic per cell and step: read y, write y'
+// (128 B, plus the halo rows) instead of 448 B for the two-stage launches.
+//
+// Moments: each stage's input moments are reduced when its input row is
+// produced (stage 1: when a y row enters the window) and read 2 iterations
+// later, when the row reaches the stage's centre; a barrier every 2
+// iterations separates every store from its reads and every last read of a
+// ring slot from its next store (kRing - 2 >= 2 iterations apart).
+//
+// The new state is written to a second buffer (neighbouring teams read y in
+// their halos while the launch runs); the state buffers swap after every
+// step.
+#include "fast_math.hpp"
+
+#include <cstdio>
+
+namespace cosenu {
+namespace {
+
+using namespace fastk;
+
+constexpr int kRing = 8, kSync = 2;
+static_assert(kRing - 2 >= kSync && kSync <= 2, "moment ring too shallow for the barrier spacing");
+constexpr int kQRing = 8 * 4 * 64; // per wavefront: [cell % 8][field][lane]
+
+// UNI: no padding bins (nvz == VP).  ca[0..3]: dt/2, dt/2, dt, dt/6.
+template <int NW, bool UNI>
+__device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, const double ca1,
+                                          const double ca2, const double ca3, double *yout, int L,
+                                          int j0, int jend, bool active, double (*rings)[kRing][NW][8],
+                                          double *qring)
+// rings: [4][kRing][NW][8] (stage s input moments); qring: the team's NW x kQRing
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wit = wave % NW;
+    const int h = lane >> 5;
+    const int v = wit * 32 + (lane & 31);
+    const int VP = A.VP, nz = A.nz;
+    const size_t zs = (size_t)8 * VP;
+    const double sig = h ? -1.0 : 1.0;
+    j0 = __builtin_amdgcn_readfirstlane(j0);
+    jend = __builtin_amdgcn_readfirstlane(jend);
+
+    const double vi = A.vz[v];
+    const bool live = UNI || v < A.nvz;
+    const bool store = active && live;
+    const int loff = 4 * h * VP + v; // this lane's first field within a z row
+    double *const qr = qring + wit * kQRing;
+
+    auto zrow = [&](int m) -> int {
+        int w = m < 0 ? m + nz : m; // m >= -12 > -nz (nz >= 16)
+        while (w >= nz)
+            w -= nz;
+        return w;
+    };
+    auto load4 = [&](const double *base, int m, double o[4]) {
+        const double *r = base + (size_t)zrow(m) * zs + loff;
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            o[f] = r[(size_t)f * VP];
+    };
+
+    const int flags = A.flags;
+    const bool coll = (flags & kFlagColl) != 0;
+    const bool vac = (flags & kFlagVac) != 0, mat = (flags & kFlagMat) != 0;
+    const bool advect = !(flags & kFlagAdvecOff);
+    const double fdfac = -vi / (12.0 * A.dz);
+    const double ko_eps = -A.ko / A.dz / 16.0;
+    const double mudv = A.mu * A.dv;
+    const double st = A.st * A.pmo, ct = A.ct * A.pmo;
+
+    // partial moments of 4 field values of this lane into ring[row % kRing]
+    auto moments = [&](double (*ring)[NW][8], int row, double p, double q, double r, double s) {
+        const double S = live ? s : 0.0;
+        const double R = live ? sig * r : 0.0;
+        const double D = live ? sig * (p - q) : 0.0;
+        const double m6[6] = {S, vi * S, R, vi * R, D, vi * D};
+        const double t = wave_sum6(m6, lane);
+        const int idx = (kMomentLane >> (4 * (lane >> 3))) & 15;
+        if ((lane & 7) == 0 && idx < 6)
+            ring[row & (kRing - 1)][wit][idx] = t;
+    };
+    // right-hand side at the window centre (position 2 of 5): z row `row`
+    // (wrapped into [0, nz)), moments in ring slot `rrow % kRing` (rrow
+    // unwrapped, as when they were stored); the terms in the order of the
+    // other FAST kernels
+    auto rhs = [&](const double w[4][5], double (*ring)[NW][8], int rrow, int row, double K[4]) {
+        const double p = w[0][2], qq = w[1][2], r = w[2][2], s = w[3][2];
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            K[f] = advect ? fdfac * ((w[f][0] - w[f][4]) - 8.0 * (w[f][1] - w[f][3])) : 0.0;
+        if (vac) {
+            K[0] += -2.0 * st * s;
+            K[1] += 2.0 * st * s;
+            K[2] += -2.0 * ct * s;
+            K[3] += 2.0 * ct * r + st * (p - qq);
+        }
+        if (mat) {
+            const double hm = sig * A.Hm[row];
+            K[2] += hm * s;
+            K[3] -= hm * r;
+        }
+        if (coll) {
+            double M[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int ww = 0; ww < NW; ww++) {
+                const double *slot = ring[rrow & (kRing - 1)][ww];
+#pragma unroll
+                for (int m = 0; m < 6; m++)
+                    M[m] += slot[m];
+            }
+            const double S = mudv * (M[0] - vi * M[1]);
+            const double R = mudv * (M[2] - vi * M[3]);
+            const double D = mudv * (M[4] - vi * M[5]);
+            const double I1 = 2.0 * (r * S - sig * s * R);
+            K[0] += I1;
+            K[1] -= I1;
+            K[2] += (qq - p) * S + sig * s * D;
+            K[3] += sig * ((p - qq) * R - r * D);
+        }
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            const double *u = w[f];
+            K[f] += ko_eps * ((u[0] + u[4]) - 4.0 * (u[1] + u[3]) + 6.0 * u[2]);
+        }
+    };
+    auto shift_in = [&](double w[4][5], const double o[4]) {
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                w[f][k] = w[f][k + 1];
+            w[f][4] = o[f];
+        }
+    };
+    auto qslot = [&](int cell, int f) -> double & { return qr[((cell & 7) * 4 + f) * 64 + lane]; };
+
+    // Iteration i: stage 1 at cell i (cells [j0-6, jend+6)), stage 2 at i-2
+    // ([j0-4, jend+4)), stage 3 at i-4 ([j0-2, jend+2)), stage 4 at i-6
+    // ([j0, jend)).  The trip count depends on L only: the same iterations,
+    // and barriers, in every team of a block.
+    const int ibeg = j0 - 6, iend = j0 + L + 6;
+    double w0[4][5], w1[4][5], w2[4][5], w3[4][5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        double t[4];
+        load4(A.yin, ibeg - 2 + k, t);
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            w0[f][k] = t[f];
+            w1[f][k] = 0.0;
+            w2[f][k] = 0.0;
+            w3[f][k] = 0.0;
+        }
+    }
+    if (coll) {
+        moments(rings[0], ibeg, w0[0][2], w0[1][2], w0[2][2], w0[3][2]);
+        moments(rings[0], ibeg + 1, w0[0][3], w0[1][3], w0[2][3], w0[3][3]);
+    }
+    double nxt[4];
+    load4(A.yin, ibeg + 3, nxt); // the row that enters the window after iteration ibeg
+    // The prologue's loads have landed before the loop (the wait-count
+    // analysis would otherwise take them as in flight inside the loop).
+    __builtin_amdgcn_s_waitcnt(0);
+
+    for (int i = ibeg; i < iend; i++) {
+        if (((i - ibeg) & (kSync - 1)) == 0) // the same iterations in every team
+            __syncthreads();
+        const int c1 = i, c2 = i - 2, c3 = i - 4, c4 = i - 6;
+        // y at the cells of stages 3 and 4 (L2: they left the window 2 and
+        // 4 iterations ago) and the next window row
+        double y3[4], y4[4], nn[4];
+        load4(A.y, c3, y3);
+        load4(A.y, c4, y4);
+        load4(A.yin, i + 4, nn);
+        if (coll)
+            moments(rings[0], i + 2, w0[0][4], w0[1][4], w0[2][4], w0[3][4]);
+
+        // stage 1 at c1: Y2 = y + (dt/2) K1; Q starts as Y2 - y
+        double o[4] = {0.0, 0.0, 0.0, 0.0};
+        if (c1 >= j0 - 6 && c1 < jend + 6) {
+            double K[4];
+            rhs(w0, rings[0], c1, zrow(c1), K);
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                o[f] = w0[f][2] + ca0 * K[f];
+                qslot(c1, f) = o[f] - w0[f][2];
+            }
+            if (coll)
+                moments(rings[1], c1, o[0], o[1], o[2], o[3]);
+        }
+        // y of stage 2's cell is the window's oldest row, before the shift
+        const double y2[4] = {w0[0][0], w0[1][0], w0[2][0], w0[3][0]};
+        shift_in(w1, o);
+        // stage 2 at c2 (window w1: rows c2-2 .. c2+2): Y3; Q += 2 (Y3 - y)
+        if (c2 >= j0 - 4 && c2 < jend + 4) {
+            double K[4];
+            rhs(w1, rings[1], c2, zrow(c2), K);
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                o[f] = y2[f] + ca1 * K[f];
+                double &q = qslot(c2, f);
+                q = fma(2.0, o[f] - y2[f], q);
+            }
+            if (coll)
+                moments(rings[2], c2, o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                o[f] = 0.0;
+        }
+        shift_in(w2, o);
+        // stage 3 at c3: Y4 = y + dt K3
+        if (c3 >= j0 - 2 && c3 < jend + 2) {
+            double K[4];
+            rhs(w2, rings[2], c3, zrow(c3), K);
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                o[f] = y3[f] + ca2 * K[f];
+            if (coll)
+                moments(rings[3], c3, o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                o[f] = 0.0;
+        }
+        shift_in(w3, o);
+        // stage 4 at c4: y' = y + ((Q + (Y4 - y))/3 + (dt/6) K4)
+        if (c4 >= j0 && c4 < jend) {
+            double K[4];
+            rhs(w3, rings[3], c4, zrow(c4), K);
+            if (store) {
+                double *out = yout + (size_t)zrow(c4) * zs + loff;
+#pragma unroll
+                for (int f = 0; f < 4; f++) {
+                    const double d = qslot(c4, f) + (w3[f][2] - y4[f]);
+                    out[(size_t)f * VP] = y4[f] + (d * (1.0 / 3.0) + ca3 * K[f]);
+                }
+            }
+        }
+        // the y window advances
+        shift_in(w0, nxt);
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            nxt[f] = nn[f];
+    }
+}
+
+template <int NW, bool UNI>
+__global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2)))
+void cosenu_quad_stage(StageArgs A, double ca0, double ca1, double ca2, double ca3, double *yout,
+                       int L, int nchunks, int nteams)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW; // teams per block
+    __shared__ double rings[TPB][4][kRing][NW][8];
+    __shared__ double qring[TPB * NW * kQRing];
+    const int team = (threadIdx.x >> 6) / NW;
+    const int g = xcd_block_order() * TPB + team;
+    double *const qr = qring + team * NW * kQRing;
+    if (nteams > 0) {
+        const bool active = g < nteams;
+        const int j0 = active ? (int)(((long)g * A.nz) / nteams) : 0;
+        const int j1 = active ? (int)(((long)(g + 1) * A.nz) / nteams) : 0;
+        quad_body<NW, UNI>(A, ca0, ca1, ca2, ca3, yout, L, j0, j1, active, rings[team], qr);
+    } else {
+        const bool active = g < nchunks;
+        const int j0 = (active ? g : 0) * L;
+        quad_body<NW, UNI>(A, ca0, ca1, ca2, ca3, yout, L, j0, j0 + L, active, rings[team], qr);
+    }
+}
+
+template <int NW>
+hipError_t launch_nw(const StageArgs &a, const double ca[4], double *yout, int L, int nteams,
+                     bool uni, hipStream_t s)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
+    int nchunks = 0, work;
+    if (nteams > 0) {
+        L = (a.nz + nteams - 1) / nteams;
+        work = nteams;
+    } else {
+        nchunks = a.nz / L;
+        work = nchunks;
+    }
+    dim3 grid((work + TPB - 1) / TPB), block(64 * NW * TPB);
+    if (uni)
+        hipLaunchKernelGGL((cosenu_quad_stage<NW, true>), grid, block, 0, s, a, ca[0], ca[1], ca[2],
+                           ca[3], yout, L, nchunks, nteams);
+    else
+        hipLaunchKernelGGL((cosenu_quad_stage<NW, false>), grid, block, 0, s, a, ca[0], ca[1],
+                           ca[2], ca[3], yout, L, nchunks, nteams);
+    return hipGetLastError();
+}
+
+template <int NW>
+int resident_teams_nw()
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
+    int nb = 0, dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&cosenu_quad_stage<NW, false>), 64 * NW * TPB, 0) !=
+        hipSuccess)
+        return 0;
+    return nb * prop.multiProcessorCount * TPB;
+}
+
+} // namespace
+
+int quad_supported(int VP, int bc, int scheme, int flags)
+{
+    return (VP == 64 || VP == 128 || VP == 256) && bc == 0 && scheme == 1 && !(flags & kFlagKO5);
+}
+
+hipError_t launch_stage_quad(const StageArgs &a, const double ca[4], double *yout, int L,
+                             int nteams, bool uniform, hipStream_t s)
+{
+    if (!quad_supported(a.VP, a.bc, a.scheme, a.flags) || a.nz < 16 || !yout)
+        return hipErrorInvalidValue;
+    if (nteams > 0) {
+        if (nteams > a.nz)
+            return hipErrorInvalidValue;
+    } else if (L < 1 || a.nz % L != 0) {
+        return hipErrorInvalidValue;
+    }
+    if (uniform && a.nvz != a.VP)
+        return hipErrorInvalidValue;
+    switch (a.VP) {
+    case 64: return launch_nw<2>(a, ca, yout, L, nteams, uniform, s);
+    case 128: return launch_nw<4>(a, ca, yout, L, nteams, uniform, s);
+    case 256: return launch_nw<8>(a, ca, yout, L, nteams, uniform, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int quad_resident_teams(int VP)
+{
+    switch (VP) {
+    case 64: return resident_teams_nw<2>();
+    case 128: return resident_teams_nw<4>();
+    case 256: return resident_teams_nw<8>();
+    default: return 0;
+    }
+}
+
+const char *stage_quad_kernel_name()
+{
+    return "cosenu_quad_stage<*, *>";
+}
+
+} // namespace cosenu

@@ -53,10 +53,11 @@ STAGE_BYTES = {1: (192, 320, 320, 256), 2: (128, 192, 192, 320), 3: (132, 196, 1
                4: (192, 256, 0, 0), 5: (128, 0, 0, 0)}
 
 
-def cpu_baseline(seconds_target: float = 12.0):
+def cpu_baseline(scheme: str = "fv", seconds_target: float = 12.0):
     """Time the reference CPU code (oracle/_ref, built from the reference's own
     headers) or, where that build is absent, the oracle restatement, on a
-    z-subsampled slice of the workload: same dz, nvz = 128, nz = 2048."""
+    z-subsampled slice of the workload: same scheme and dz, nvz = 128,
+    nz = 2048."""
     import numpy as np
     import oracle as O
     from cosenu_amd.initial import seeded_ic
@@ -66,9 +67,9 @@ def cpu_baseline(seconds_target: float = 12.0):
     dz = 3000.0 / 65536
     nz = 2048
     p = O.RunParams(nz=nz, nvz=128, z0=-nz * dz / 2, z1=nz * dz / 2, flags=O.VAC | O.COLL,
-                    omega=0.0, theta=1e-6)
+                    omega=0.0, theta=1e-6, scheme=O.FD if scheme == "fd" else O.FV)
     G, bG, s = seeded_ic(p, 0.6, 0.53, 0.92, seed=1)
-    use_ref = O.ref_available("fv_vc")
+    use_ref = O.ref_available(O.ref_variant(p))
     if use_ref:
         import tempfile
         run = O.RefRun(p, tempfile.mkdtemp(prefix="cosenu_ref_"))
@@ -77,10 +78,13 @@ def cpu_baseline(seconds_target: float = 12.0):
     else:
         fn = lambda st, n: O.step(p, st, n, threads=threads)
         kind = "port"
-    import contextlib
-    import io
-    sink = io.StringIO()
-    with contextlib.redirect_stdout(sink):
+    # the reference prints a banner per run from C++: keep stdout to the one
+    # JSON line by pointing file descriptor 1 at /dev/null meanwhile
+    sys.stdout.flush()
+    saved = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
         t0 = time.perf_counter()
         fn(s, 1)
         t1 = time.perf_counter()
@@ -89,10 +93,15 @@ def cpu_baseline(seconds_target: float = 12.0):
         t0 = time.perf_counter()
         fn(s, n)
         el = time.perf_counter() - t0
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(devnull)
     cells = nz * 128
     return {"value": cells * n / el, "unit": "cell-updates/s", "cores": threads, "kind": kind,
-            "sample": f"{n} RK4 steps of FV-WENO7 (vac+coll) on nz={nz} x nvz=128, same dz as "
-                      f"the workload, {threads} OpenMP threads, {el:.1f} s"}
+            "sample": f"{n} RK4 steps of {'FD' if scheme == 'fd' else 'FV-WENO7'} (vac+coll) on "
+                      f"nz={nz} x nvz=128, same dz as the workload, {threads} OpenMP threads, "
+                      f"{el:.1f} s"}
 
 
 def main():
@@ -247,7 +256,7 @@ def main():
     }
     sim.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(args.scheme)
     print(json.dumps(out), flush=True)
 
 

@@ -575,13 +575,19 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
         } else if (c->stream_ok && c->nz >= 16 && (long)c->nz * (c->VP / 32) <= 2048) {
             // small problems (at most 2 wavefronts per SIMD in all): the
             // barrier-free kernel in chunks of 2 rows (nz = 1024, nvz = 64:
-            // 27 us per FV step against 45 us for two-stage ranges)
+            // 27 us per FV step against 45 us for two-stage ranges; FD 20 us
+            // against 28 us for four-stage ranges)
             c->variant = 3;
             chunk = c->nz % 2 ? 1 : 2;
         } else {
-            // two stages per launch where it applies, else the split-lane
-            // kernel; contiguous z ranges, one team per resident slot
-            c->variant = pair_supported(c->VP, p->bc) && c->nz >= 16 ? 4 : 2;
+            // FD with 3rd-order dissipation: four stages per launch (nz =
+            // 65536, nvz = 128: 0.54 ms per step against 0.84 ms for two-stage
+            // launches); else two stages per launch where it applies, else the
+            // split-lane kernel; contiguous z ranges, one team per resident slot
+            if (quad_supported(c->VP, p->bc, p->scheme, p->flags) && c->nz >= 16)
+                c->variant = 5;
+            else
+                c->variant = pair_supported(c->VP, p->bc) && c->nz >= 16 ? 4 : 2;
             chunk = -1;
         }
         int ev = env_int("COSENU_FAST_VARIANT", 0), ez = env_int("COSENU_ZCHUNK", 0);

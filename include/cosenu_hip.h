@@ -81,11 +81,13 @@ typedef struct cosenu_info {
     long long steps_done;
     size_t device_bytes; /* device memory held by the context */
     int z_chunk;         /* FAST mode: z cells per streaming work item */
-    int fast_variant;    /* FAST mode kernel: 4 = two RK4 stages per launch
-                            (default), 3 = barrier-free (default for small
-                            problems), 2 = one stage per launch, two lanes per
-                            bin (nu / nubar halves), 1 = one lane per bin;
-                            0 in STRICT */
+    int fast_variant;    /* FAST mode kernel: 5 = all four RK4 stages per
+                            launch (default for FD with 3rd-order dissipation,
+                            periodic, nvz <= 256), 4 = two RK4 stages per
+                            launch (default otherwise), 3 = barrier-free
+                            (default for small problems), 2 = one stage per
+                            launch, two lanes per bin (nu / nubar halves), 1 =
+                            one lane per bin; 0 in STRICT */
     int range_teams;     /* FAST mode: > 0 when each of this many teams streams
                             one contiguous z range (z_chunk = its length) */
 } cosenu_info;
@@ -180,7 +182,8 @@ void cosenu_host_free(void *p);
  * step since timing was enabled, counts[k] = launches timed: launch k = RK
  * stage k+1 for the one-stage kernels; with two stages per launch (FAST
  * variant 4) launch 0 = stages 1+2, launch 1 = stages 3+4 and slots 2, 3
- * stay empty.  Enabling clears.  While enabled, steps are launched one by
+ * stay empty; with all four in one launch (variant 5) slot 0 only.
+ * Enabling clears.  While enabled, steps are launched one by
  * one (no graph replay).  cosenu_stage_kernel_name(ctx, k) names the kernel
  * of launch k. */
 int cosenu_set_timing(cosenu_ctx *ctx, int enable);

@@ -195,3 +195,25 @@ def test_quad_autotune_considers_four_stage_kernel(gpu, tmp_path, monkeypatch):
             sim.step(4)
             outs[var] = sim.download()
     assert _rel(outs["5"], outs["4"]) <= 1e-12
+
+
+def test_quad_is_the_fd_default_and_times_one_launch(gpu, monkeypatch):
+    """Without a forced variant FD runs of a size that fills the device take
+    the four-stage kernel; timing reports one launch per step."""
+    monkeypatch.delenv("COSENU_FAST_VARIANT", raising=False)
+    monkeypatch.delenv("COSENU_ZCHUNK", raising=False)
+    monkeypatch.delenv("COSENU_TUNE_FILE", raising=False)
+    po, pc = _quad(8192, 128)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=51, amp=1e-3)
+    with C.Simulation(pc) as sim:
+        info = sim.info()
+        assert info["fast_variant"] == 5 and info["range_teams"] > 0
+        sim.upload(s)
+        sim.set_timing(True)
+        sim.step(5)
+        ms, cnt = sim.timing()
+        names = [sim.stage_kernel_name(k) for k in range(4)]
+        out = sim.download()
+    assert cnt == [5, 0, 0, 0] and ms[0] > 0
+    assert "quad" in names[0] and not any(names[1:])
+    assert _rel(out, O.step(po, s, 5, moment_form=True, threads=8)) <= 1e-10

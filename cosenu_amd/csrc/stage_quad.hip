@@ -14,11 +14,17 @@
 //
 // RK4 (nuosc.hpp:320-348) in the stage-value form of the other FAST kernels:
 //   Y2 = y + (dt/2) K1, Y3 = y + (dt/2) K2, Y4 = y + dt K3,
-//   y' = y + ((Q + (Y4 - y))/3 + (dt/6) K4), Q = (Y2 - y) + 2 (Y3 - y).
-// Q is accumulated per cell in a wavefront-private LDS ring (8 cells); y at
-// the cells of stages 3 and 4 is read again (4 and 6 rows after it left the
-// first window: from L2).  HBM traffic per cell and step: read y, write y'
-// (128 B, plus the halo rows) instead of 448 B for the two-stage launches.
+//   T = y + (Q + (Y4 - y))/3, Q = (Y2 - y) + 2 (Y3 - y),  y' = T + (dt/6) K4.
+// Stage 2 holds y (the row leaving the first window), Y2 (its window's
+// centre) and Y3 of its cell, forms Q and stores Q and y in wavefront-private
+// LDS rings; stage 3 takes y from its ring and replaces Q by T, which is all
+// stage 4 needs.  A ring slot lives from stage 2 (iteration c + 2) to stage 4
+// (c + 6): 5 slots for Q / T, 3 for y (c + 2 .. c + 4), 16 KB per wavefront.
+// HBM traffic per cell and step: read y, write y' (128 B, plus the halo
+// rows) instead of 448 B for the two-stage launches.  (T rounds a
+// state-sized value once more than the two-stage kernel's y + (d/3 +
+// (dt/6) K4): unbiased, ~1e-14 relative after 20000 steps, and a zero
+// right-hand side still leaves the state unchanged.)
 //
 // Moments: each stage's input moments are reduced when its input row is
 // produced (stage 1: when a y row enters the window) and read 2 iterations
@@ -40,7 +46,8 @@ using namespace fastk;
 
 constexpr int kRing = 8, kSync = 2;
 static_assert(kRing - 2 >= kSync && kSync <= 2, "moment ring too shallow for the barrier spacing");
-constexpr int kQRing = 8 * 4 * 64; // per wavefront: [cell % 8][field][lane]
+constexpr int kTRing = 5, kYRing = 3; // ring slots per wavefront: Q / T and y
+constexpr int kQRing = (kTRing + kYRing) * 4 * 64; // doubles per wavefront: [slot][field][lane]
 
 // UNI: no padding bins (nvz == VP).  ca[0..3]: dt/2, dt/2, dt, dt/6.
 template <int NW, bool UNI>
@@ -152,7 +159,14 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             w[f][4] = o[f];
         }
     };
-    auto qslot = [&](int cell, int f) -> double & { return qr[((cell & 7) * 4 + f) * 64 + lane]; };
+    // ring slots by cell (cells >= ibeg - 6, see below)
+    const int cbase = j0 - 12;
+    auto tslot = [&](int cell, int f) -> double & {
+        return qr[(((cell - cbase) % kTRing) * 4 + f) * 64 + lane];
+    };
+    auto yslot = [&](int cell, int f) -> double & {
+        return qr[((kTRing + (cell - cbase) % kYRing) * 4 + f) * 64 + lane];
+    };
 
     // Iteration i: stage 1 at cell i (cells [j0-6, jend+6)), stage 2 at i-2
     // ([j0-4, jend+4)), stage 3 at i-4 ([j0-2, jend+2)), stage 4 at i-6
@@ -186,40 +200,35 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         if (((i - ibeg) & (kSync - 1)) == 0) // the same iterations in every team
             __syncthreads();
         const int c1 = i, c2 = i - 2, c3 = i - 4, c4 = i - 6;
-        // y at the cells of stages 3 and 4 (L2: they left the window 2 and
-        // 4 iterations ago) and the next window row
-        double y3[4], y4[4], nn[4];
-        load4(A.y, c3, y3);
-        load4(A.y, c4, y4);
+        // the next window row
+        double nn[4];
         load4(A.yin, i + 4, nn);
         if (coll)
             moments(rings[0], i + 2, w0[0][4], w0[1][4], w0[2][4], w0[3][4]);
 
-        // stage 1 at c1: Y2 = y + (dt/2) K1; Q starts as Y2 - y
+        // stage 1 at c1: Y2 = y + (dt/2) K1
         double o[4] = {0.0, 0.0, 0.0, 0.0};
         if (c1 >= j0 - 6 && c1 < jend + 6) {
             double K[4];
             rhs(w0, rings[0], c1, zrow(c1), K);
 #pragma unroll
-            for (int f = 0; f < 4; f++) {
+            for (int f = 0; f < 4; f++)
                 o[f] = w0[f][2] + ca0 * K[f];
-                qslot(c1, f) = o[f] - w0[f][2];
-            }
             if (coll)
                 moments(rings[1], c1, o[0], o[1], o[2], o[3]);
         }
         // y of stage 2's cell is the window's oldest row, before the shift
         const double y2[4] = {w0[0][0], w0[1][0], w0[2][0], w0[3][0]};
         shift_in(w1, o);
-        // stage 2 at c2 (window w1: rows c2-2 .. c2+2): Y3; Q += 2 (Y3 - y)
+        // stage 2 at c2 (window w1: rows c2-2 .. c2+2): Y3; Q and y to the rings
         if (c2 >= j0 - 4 && c2 < jend + 4) {
             double K[4];
             rhs(w1, rings[1], c2, zrow(c2), K);
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 o[f] = y2[f] + ca1 * K[f];
-                double &q = qslot(c2, f);
-                q = fma(2.0, o[f] - y2[f], q);
+                tslot(c2, f) = fma(2.0, o[f] - y2[f], w1[f][2] - y2[f]);
+                yslot(c2, f) = y2[f];
             }
             if (coll)
                 moments(rings[2], c2, o[0], o[1], o[2], o[3]);
@@ -229,13 +238,17 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 o[f] = 0.0;
         }
         shift_in(w2, o);
-        // stage 3 at c3: Y4 = y + dt K3
+        // stage 3 at c3: Y4 = y + dt K3; T = y + (Q + (Y4 - y))/3 replaces Q
         if (c3 >= j0 - 2 && c3 < jend + 2) {
             double K[4];
             rhs(w2, rings[2], c3, zrow(c3), K);
 #pragma unroll
-            for (int f = 0; f < 4; f++)
-                o[f] = y3[f] + ca2 * K[f];
+            for (int f = 0; f < 4; f++) {
+                const double yv = yslot(c3, f);
+                o[f] = yv + ca2 * K[f];
+                double &t = tslot(c3, f);
+                t = yv + (t + (o[f] - yv)) * (1.0 / 3.0);
+            }
             if (coll)
                 moments(rings[3], c3, o[0], o[1], o[2], o[3]);
         } else {
@@ -244,17 +257,15 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 o[f] = 0.0;
         }
         shift_in(w3, o);
-        // stage 4 at c4: y' = y + ((Q + (Y4 - y))/3 + (dt/6) K4)
+        // stage 4 at c4: y' = T + (dt/6) K4
         if (c4 >= j0 && c4 < jend) {
             double K[4];
             rhs(w3, rings[3], c4, zrow(c4), K);
             if (store) {
                 double *out = yout + (size_t)zrow(c4) * zs + loff;
 #pragma unroll
-                for (int f = 0; f < 4; f++) {
-                    const double d = qslot(c4, f) + (w3[f][2] - y4[f]);
-                    out[(size_t)f * VP] = y4[f] + (d * (1.0 / 3.0) + ca3 * K[f]);
-                }
+                for (int f = 0; f < 4; f++)
+                    out[(size_t)f * VP] = tslot(c4, f) + ca3 * K[f];
             }
         }
         // the y window advances

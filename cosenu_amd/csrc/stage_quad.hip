@@ -28,9 +28,9 @@
 //
 // Moments: each stage's input moments are reduced when its input row is
 // produced (stage 1: when a y row enters the window) and read 2 iterations
-// later, when the row reaches the stage's centre; a barrier every 2
-// iterations separates every store from its reads and every last read of a
-// ring slot from its next store (kRing - 2 >= 2 iterations apart).
+// later, when the row reaches the stage's centre; as no two consecutive
+// iterations lack a barrier, one separates every store from its reads and
+// every last read of a ring slot from its next store (6 iterations apart).
 //
 // The new state is written to a second buffer (neighbouring teams read y in
 // their halos while the launch runs); the state buffers swap after every
@@ -38,14 +38,22 @@
 #include "fast_math.hpp"
 
 #include <cstdio>
+#include <type_traits>
 
 namespace cosenu {
 namespace {
 
 using namespace fastk;
 
-constexpr int kRing = 8, kSync = 2;
-static_assert(kRing - 2 >= kSync && kSync <= 2, "moment ring too shallow for the barrier spacing");
+// The z loop runs in groups of 5 iterations (the window length), so that a
+// window shift is a change of which register holds which row.  Barriers at
+// iterations 0, 2, 4 of each group: never two iterations in a row without
+// one, which is what the moment rings need (a row's moments are stored 2
+// iterations before they are read; a slot is reused kRing - 2 = 6 iterations
+// after its last read).
+constexpr int kRing = 8;
+template <int T>
+using Rot = std::integral_constant<int, T>;
 constexpr int kTRing = 5, kYRing = 3; // ring slots per wavefront: Q / T and y
 constexpr int kQRing = (kTRing + kYRing) * 4 * 64; // doubles per wavefront: [slot][field][lane]
 
@@ -90,10 +98,11 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
     const bool coll = (flags & kFlagColl) != 0;
     const bool vac = (flags & kFlagVac) != 0, mat = (flags & kFlagMat) != 0;
     const bool advect = !(flags & kFlagAdvecOff);
-    const double fdfac = -vi / (12.0 * A.dz);
+    // without advection the stencil is multiplied by 0 (no per-value select)
+    const double fdfac = advect ? -vi / (12.0 * A.dz) : 0.0;
     const double ko_eps = -A.ko / A.dz / 16.0;
     const double mudv = A.mu * A.dv;
-    const double st = A.st * A.pmo, ct = A.ct * A.pmo;
+    const double st = A.st * A.pmo, ct = A.ct * A.pmo, st2 = 2.0 * st, ct2 = 2.0 * ct;
 
     // partial moments of 4 field values of this lane into ring[row % kRing]
     auto moments = [&](double (*ring)[NW][8], int row, double p, double q, double r, double s) {
@@ -106,25 +115,32 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         if ((lane & 7) == 0 && idx < 6)
             ring[row & (kRing - 1)][wit][idx] = t;
     };
-    // right-hand side at the window centre (position 2 of 5): z row `row`
-    // (wrapped into [0, nz)), moments in ring slot `rrow % kRing` (rrow
-    // unwrapped, as when they were stored); the terms in the order of the
-    // other FAST kernels
-    auto rhs = [&](const double w[4][5], double (*ring)[NW][8], int rrow, int row, double K[4]) {
-        const double p = w[0][2], qq = w[1][2], r = w[2][2], s = w[3][2];
+    // right-hand side at the window centre: z row `row` (wrapped into [0,
+    // nz)), moments in ring slot `rrow % kRing` (rrow unwrapped, as when they
+    // were stored); the terms in the order of the other FAST kernels.  The
+    // window's rows r-2 .. r+2 are in w[f][(RT + k) % 5], k = 0..4.  Every
+    // fused multiply-add is written out (this file is built without
+    // contraction), so the 5 unrolled copies of the z step evaluate a cell
+    // identically whichever copy it falls in.
+    auto rhs = [&](const double (&w)[4][5], auto rot, double (*ring)[NW][8], int rrow, int row,
+                   double K[4]) __attribute__((always_inline)) {
+        constexpr int RT = decltype(rot)::value;
+        constexpr int k0 = RT % 5, k1 = (RT + 1) % 5, k2 = (RT + 2) % 5, k3 = (RT + 3) % 5,
+                      k4 = (RT + 4) % 5;
+        const double p = w[0][k2], qq = w[1][k2], r = w[2][k2], s = w[3][k2];
 #pragma unroll
         for (int f = 0; f < 4; f++)
-            K[f] = advect ? fdfac * ((w[f][0] - w[f][4]) - 8.0 * (w[f][1] - w[f][3])) : 0.0;
+            K[f] = fdfac * fma(-8.0, w[f][k1] - w[f][k3], w[f][k0] - w[f][k4]);
         if (vac) {
-            K[0] += -2.0 * st * s;
-            K[1] += 2.0 * st * s;
-            K[2] += -2.0 * ct * s;
-            K[3] += 2.0 * ct * r + st * (p - qq);
+            K[0] = fma(-st2, s, K[0]);
+            K[1] = fma(st2, s, K[1]);
+            K[2] = fma(-ct2, s, K[2]);
+            K[3] = fma(ct2, r, fma(st, p - qq, K[3]));
         }
         if (mat) {
             const double hm = sig * A.Hm[row];
-            K[2] += hm * s;
-            K[3] -= hm * r;
+            K[2] = fma(hm, s, K[2]);
+            K[3] = fma(-hm, r, K[3]);
         }
         if (coll) {
             double M[6] = {0, 0, 0, 0, 0, 0};
@@ -135,28 +151,20 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 for (int m = 0; m < 6; m++)
                     M[m] += slot[m];
             }
-            const double S = mudv * (M[0] - vi * M[1]);
-            const double R = mudv * (M[2] - vi * M[3]);
-            const double D = mudv * (M[4] - vi * M[5]);
-            const double I1 = 2.0 * (r * S - sig * s * R);
+            const double S = mudv * fma(-vi, M[1], M[0]);
+            const double R = mudv * fma(-vi, M[3], M[2]);
+            const double D = mudv * fma(-vi, M[5], M[4]);
+            const double ss = sig * s;
+            const double I1 = 2.0 * fma(r, S, -(ss * R));
             K[0] += I1;
             K[1] -= I1;
-            K[2] += (qq - p) * S + sig * s * D;
-            K[3] += sig * ((p - qq) * R - r * D);
+            K[2] = fma(qq - p, S, fma(ss, D, K[2]));
+            K[3] = fma(sig, fma(p - qq, R, -(r * D)), K[3]);
         }
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             const double *u = w[f];
-            K[f] += ko_eps * ((u[0] + u[4]) - 4.0 * (u[1] + u[3]) + 6.0 * u[2]);
-        }
-    };
-    auto shift_in = [&](double w[4][5], const double o[4]) {
-#pragma unroll
-        for (int f = 0; f < 4; f++) {
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                w[f][k] = w[f][k + 1];
-            w[f][4] = o[f];
+            K[f] = fma(ko_eps, fma(6.0, u[k2], fma(-4.0, u[k1] + u[k3], u[k0] + u[k4])), K[f]);
         }
     };
     // ring slots by cell (cells >= ibeg - 6, see below)
@@ -196,39 +204,47 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
     // analysis would otherwise take them as in flight inside the loop).
     __builtin_amdgcn_s_waitcnt(0);
 
-    for (int i = ibeg; i < iend; i++) {
-        if (((i - ibeg) & (kSync - 1)) == 0) // the same iterations in every team
+    // Iteration i = ibeg + 5 m + T.  Window rows: w0 holds y rows i-2 .. i+2
+    // at w0[f][(T + k) % 5]; w1, w2, w3 hold stage outputs and are shifted
+    // before their stage runs, after which their rows c-2 .. c+2 are at
+    // [(T + 1 + k) % 5].  A shift overwrites slot T % 5 (the oldest row).
+    auto step = [&](const int i, auto tc) __attribute__((always_inline)) {
+        constexpr int T = decltype(tc)::value;
+        constexpr int old = T % 5, cen0 = (T + 2) % 5, top0 = (T + 4) % 5, cen1 = (T + 3) % 5;
+        if (T % 2 == 0) // the same iterations in every team
             __syncthreads();
         const int c1 = i, c2 = i - 2, c3 = i - 4, c4 = i - 6;
         // the next window row
         double nn[4];
         load4(A.yin, i + 4, nn);
         if (coll)
-            moments(rings[0], i + 2, w0[0][4], w0[1][4], w0[2][4], w0[3][4]);
+            moments(rings[0], i + 2, w0[0][top0], w0[1][top0], w0[2][top0], w0[3][top0]);
 
         // stage 1 at c1: Y2 = y + (dt/2) K1
         double o[4] = {0.0, 0.0, 0.0, 0.0};
         if (c1 >= j0 - 6 && c1 < jend + 6) {
             double K[4];
-            rhs(w0, rings[0], c1, zrow(c1), K);
+            rhs(w0, Rot<T>{}, rings[0], c1, zrow(c1), K);
 #pragma unroll
             for (int f = 0; f < 4; f++)
-                o[f] = w0[f][2] + ca0 * K[f];
+                o[f] = fma(ca0, K[f], w0[f][cen0]);
             if (coll)
                 moments(rings[1], c1, o[0], o[1], o[2], o[3]);
         }
-        // y of stage 2's cell is the window's oldest row, before the shift
-        const double y2[4] = {w0[0][0], w0[1][0], w0[2][0], w0[3][0]};
-        shift_in(w1, o);
-        // stage 2 at c2 (window w1: rows c2-2 .. c2+2): Y3; Q and y to the rings
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            w1[f][old] = o[f];
+        // stage 2 at c2: Y3; Q and y to the rings.  y of this cell is the y
+        // window's oldest row (shifted out at the end of the iteration).
         if (c2 >= j0 - 4 && c2 < jend + 4) {
             double K[4];
-            rhs(w1, rings[1], c2, zrow(c2), K);
+            rhs(w1, Rot<T + 1>{}, rings[1], c2, zrow(c2), K);
 #pragma unroll
             for (int f = 0; f < 4; f++) {
-                o[f] = y2[f] + ca1 * K[f];
-                tslot(c2, f) = fma(2.0, o[f] - y2[f], w1[f][2] - y2[f]);
-                yslot(c2, f) = y2[f];
+                const double y2 = w0[f][old];
+                o[f] = fma(ca1, K[f], y2);
+                tslot(c2, f) = fma(2.0, o[f] - y2, w1[f][cen1] - y2);
+                yslot(c2, f) = y2;
             }
             if (coll)
                 moments(rings[2], c2, o[0], o[1], o[2], o[3]);
@@ -237,17 +253,19 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             for (int f = 0; f < 4; f++)
                 o[f] = 0.0;
         }
-        shift_in(w2, o);
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            w2[f][old] = o[f];
         // stage 3 at c3: Y4 = y + dt K3; T = y + (Q + (Y4 - y))/3 replaces Q
         if (c3 >= j0 - 2 && c3 < jend + 2) {
             double K[4];
-            rhs(w2, rings[2], c3, zrow(c3), K);
+            rhs(w2, Rot<T + 1>{}, rings[2], c3, zrow(c3), K);
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 const double yv = yslot(c3, f);
-                o[f] = yv + ca2 * K[f];
+                o[f] = fma(ca2, K[f], yv);
                 double &t = tslot(c3, f);
-                t = yv + (t + (o[f] - yv)) * (1.0 / 3.0);
+                t = fma(t + (o[f] - yv), 1.0 / 3.0, yv);
             }
             if (coll)
                 moments(rings[3], c3, o[0], o[1], o[2], o[3]);
@@ -256,23 +274,42 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             for (int f = 0; f < 4; f++)
                 o[f] = 0.0;
         }
-        shift_in(w3, o);
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            w3[f][old] = o[f];
         // stage 4 at c4: y' = T + (dt/6) K4
         if (c4 >= j0 && c4 < jend) {
             double K[4];
-            rhs(w3, rings[3], c4, zrow(c4), K);
+            rhs(w3, Rot<T + 1>{}, rings[3], c4, zrow(c4), K);
             if (store) {
                 double *out = yout + (size_t)zrow(c4) * zs + loff;
 #pragma unroll
                 for (int f = 0; f < 4; f++)
-                    out[(size_t)f * VP] = tslot(c4, f) + ca3 * K[f];
+                    out[(size_t)f * VP] = fma(ca3, K[f], tslot(c4, f));
             }
         }
         // the y window advances
-        shift_in(w0, nxt);
 #pragma unroll
-        for (int f = 0; f < 4; f++)
+        for (int f = 0; f < 4; f++) {
+            w0[f][old] = nxt[f];
             nxt[f] = nn[f];
+        }
+    };
+
+    for (int i = ibeg; i < iend; i += 5) {
+        step(i, Rot<0>{});
+        if (i + 1 >= iend)
+            break;
+        step(i + 1, Rot<1>{});
+        if (i + 2 >= iend)
+            break;
+        step(i + 2, Rot<2>{});
+        if (i + 3 >= iend)
+            break;
+        step(i + 3, Rot<3>{});
+        if (i + 4 >= iend)
+            break;
+        step(i + 4, Rot<4>{});
     }
 }
 

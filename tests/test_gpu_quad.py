@@ -97,12 +97,16 @@ def test_quad_graph_eager_and_call_pattern_identical(gpu, monkeypatch):
         assert np.array_equal(outs[0], o)
 
 
-@pytest.mark.parametrize("chunk", ["24", "15", "2", "1", "-1", "-2"])
-def test_quad_launch_shapes_identical(gpu, monkeypatch, chunk):
+@pytest.mark.parametrize("nvz,chunk", [(128, "24"), (128, "15"), (128, "2"), (128, "1"),
+                                       (128, "-1"), (128, "-2"), (40, "15"), (40, "-1"),
+                                       (256, "15"), (256, "-1")])
+def test_quad_launch_shapes_identical(gpu, monkeypatch, nvz, chunk):
     """Chunks of 48 rows against chunks of 24, 15, 2 and 1 rows and contiguous
-    ranges: bit for bit (every cell's four stages run the same arithmetic
-    whichever team computes them)."""
-    po, pc = _quad(480, 128)
+    ranges: bit for bit.  A cell falls in a different one of the z loop's 5
+    unrolled steps depending on where its team's range starts; every copy
+    rounds it alike (padding bins: nvz = 40; 8 wavefronts per team:
+    nvz = 256)."""
+    po, pc = _quad(480, nvz)
     _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=45, amp=1e-3)
     outs = []
     for ch in ("48", chunk):

@@ -1099,6 +1099,7 @@ struct Group {
     int2 *d_items = nullptr;
     int nitems = 0;
     bool uniform = false; // every run: nvz == VP, groups of 32 bins of one sign of v
+    bool quad = false;    // four stages per launch (FD, 3rd-order dissipation), else two
 };
 
 } // namespace
@@ -1157,9 +1158,17 @@ int batch_enqueue_step(cosenu_batch *b, hipStream_t s)
 {
     if (b->pairs) {
         for (int pair = 0; pair < 2; pair++)
-            for (Group &g : b->groups)
-                HIP_TRY(launch_stage_pair_batch(b->parity ? g.d_runs_sw : g.d_runs, g.d_items,
-                                                g.nitems, g.VP, g.scheme, g.L, pair, g.uniform, s));
+            for (Group &g : b->groups) {
+                const BatchRun *runs = b->parity ? g.d_runs_sw : g.d_runs;
+                if (g.quad) {
+                    if (pair == 0)
+                        HIP_TRY(launch_stage_quad_batch(runs, g.d_items, g.nitems, g.VP, g.L,
+                                                        g.uniform, s));
+                } else {
+                    HIP_TRY(launch_stage_pair_batch(runs, g.d_items, g.nitems, g.VP, g.scheme,
+                                                    g.L, pair, g.uniform, s));
+                }
+            }
         b->parity ^= 1;
         for (int r = 0; r < b->nruns; r++)
             b->views[r].y = b->parity ? b->run_sp[r] : b->run_y[r];
@@ -1188,8 +1197,10 @@ int group_chunk(const cosenu_batch *b, const Group &g)
             rows += b->P[r].nz;
         if (b->pairs) {
             // one work item per resident team, 32 to 512 cells (each item
-            // recomputes 8 rows of halo for its second stage)
-            const long target = std::max(1, pair_resident_teams(g.VP, g.scheme));
+            // recomputes 8 rows of halo for its second stage, 12 over three
+            // stages with four stages per launch)
+            const long target = std::max(1, g.quad ? quad_resident_teams(g.VP)
+                                                   : pair_resident_teams(g.VP, g.scheme));
             want = (int)std::max(32L, std::min(512L, rows / target));
         } else {
             // about two work items per resident team, chunks of 8 to 32
@@ -1291,12 +1302,20 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
         g.n_hm += runs[r].nz;
         b->cells += (long long)runs[r].nz * runs[r].nvz;
     }
-    // two stages per launch unless COSENU_BATCH_VARIANT=2 (one stage per launch)
-    b->pairs = env_int("COSENU_BATCH_VARIANT", 4) != 2;
+    // two stages per launch unless COSENU_BATCH_VARIANT=2 (one stage per
+    // launch); groups of FD runs with 3rd-order dissipation four stages per
+    // launch unless COSENU_BATCH_VARIANT=4
+    const int bvar = env_int("COSENU_BATCH_VARIANT", 5);
+    b->pairs = bvar != 2;
     for (int r = 0; r < nruns; r++)
         b->pairs = b->pairs && runs[r].nz >= 16;
     for (const Group &g : b->groups)
         b->pairs = b->pairs && pair_supported(g.VP, COSENU_PERIODIC);
+    for (Group &g : b->groups) {
+        g.quad = b->pairs && bvar != 4;
+        for (int r : g.runs)
+            g.quad = g.quad && quad_supported(g.VP, COSENU_PERIODIC, runs[r].scheme, runs[r].flags);
+    }
     int rc = 0;
     for (Group &g : b->groups) {
         if (!rc) rc = dev_alloc(b->bytes, &g.y, g.n_state);
@@ -1513,6 +1532,9 @@ int cosenu_batch_get_info(cosenu_batch *b, cosenu_batch_info *info)
         info->work_items += g.nitems;
     info->z_chunk = b->groups.empty() ? 0 : b->groups[0].L;
     info->fast_variant = b->pairs ? 4 : 2;
+    for (const Group &g : b->groups)
+        if (g.quad)
+            info->fast_variant = 5;
     return 0;
 }
 
@@ -1559,7 +1581,7 @@ int cosenu_batch_autotune(const cosenu_params *runs, int nruns, int *z_chunk, do
     }
     if (!rc)
         for (const Group &g : b->groups) // batches created afterwards use it
-            tune_store(g.VP, g.scheme, -1, {b->pairs ? 4 : 2, bl, best});
+            tune_store(g.VP, g.scheme, -1, {g.quad ? 5 : b->pairs ? 4 : 2, bl, best});
     if (e0)
         (void)hipEventDestroy(e0);
     if (e1)

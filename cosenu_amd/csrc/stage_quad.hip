@@ -336,6 +336,43 @@ void cosenu_quad_stage(StageArgs A, double ca0, double ca1, double ca2, double c
     }
 }
 
+// Batched runs: work item g = (run, first z cell) of a packed list; every
+// item of a launch has the same length L (a run's last item may hold fewer
+// rows), so every team of a block runs the same iterations and barriers.
+// The new state goes to the run's spare buffer (acc).
+template <int NW, bool UNI>
+__global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2)))
+void cosenu_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int L)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
+    __shared__ double rings[TPB][4][kRing][NW][8];
+    __shared__ double qring[TPB * NW * kQRing];
+    const int team = (threadIdx.x >> 6) / NW;
+    const int item = xcd_block_order() * TPB + team;
+    const bool active = item < nitems;
+    const int2 it = items[active ? item : 0];
+    const BatchRun &r = runs[it.x];
+    const StageArgs A = batch_stage_args(r, 1, true);
+    const int jend = min(it.y + L, r.nz);
+    quad_body<NW, UNI>(A, r.a[0], r.a[1], r.a[2], r.a[3], r.acc, L, it.y, jend, active,
+                       rings[team], qring + team * NW * kQRing);
+}
+
+template <int NW>
+hipError_t launch_batch_nw(const BatchRun *runs, const int2 *items, int nitems, int L, bool uni,
+                           hipStream_t s)
+{
+    constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
+    if (nitems <= 0)
+        return hipSuccess;
+    dim3 grid((nitems + TPB - 1) / TPB), block(64 * NW * TPB);
+    if (uni)
+        hipLaunchKernelGGL((cosenu_quad_batch<NW, true>), grid, block, 0, s, runs, items, nitems, L);
+    else
+        hipLaunchKernelGGL((cosenu_quad_batch<NW, false>), grid, block, 0, s, runs, items, nitems, L);
+    return hipGetLastError();
+}
+
 template <int NW>
 hipError_t launch_nw(const StageArgs &a, const double ca[4], double *yout, int L, int nteams,
                      bool uni, hipStream_t s)
@@ -398,6 +435,19 @@ hipError_t launch_stage_quad(const StageArgs &a, const double ca[4], double *you
     case 64: return launch_nw<2>(a, ca, yout, L, nteams, uniform, s);
     case 128: return launch_nw<4>(a, ca, yout, L, nteams, uniform, s);
     case 256: return launch_nw<8>(a, ca, yout, L, nteams, uniform, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_stage_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
+                                   int L, bool uniform, hipStream_t s)
+{
+    if (L < 1)
+        return hipErrorInvalidValue;
+    switch (VP) {
+    case 64: return launch_batch_nw<2>(runs, items, nitems, L, uniform, s);
+    case 128: return launch_batch_nw<4>(runs, items, nitems, L, uniform, s);
+    case 256: return launch_batch_nw<8>(runs, items, nitems, L, uniform, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -219,7 +219,9 @@ typedef struct cosenu_batch_info {
     size_t device_bytes;
     long long work_items; /* (run, z-chunk) items per launch */
     int z_chunk;          /* chunk length of the first group */
-    int fast_variant;     /* 4: two RK4 stages per launch (default), 2: one */
+    int fast_variant;     /* 4: two RK4 stages per launch (default), 2: one;
+                             5: groups of FD runs with 3rd-order dissipation
+                             take all four stages per launch (the others two) */
 } cosenu_batch_info;
 
 int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out);

@@ -183,12 +183,13 @@ def test_quad_autotune_considers_four_stage_kernel(gpu, tmp_path, monkeypatch):
     table = tmp_path / "tune.txt"
     monkeypatch.setenv("COSENU_TUNE_FILE", str(table))
     monkeypatch.delenv("COSENU_FAST_VARIANT", raising=False)
-    p = C.RunParams(nz=4096, nvz=128, scheme=C.FD, mode=C.FAST)
+    # (a shape no other test uses: the tuned entry stays in the process's table)
+    p = C.RunParams(nz=3072, nvz=128, scheme=C.FD, mode=C.FAST)
     v, L, ms = C.autotune(p)
     assert v in (1, 2, 3, 4, 5) and ms > 0
     with C.Simulation(p) as sim:
         assert sim.info()["fast_variant"] == v
-    po = O.RunParams(nz=4096, nvz=128, scheme=O.FD)
+    po = O.RunParams(nz=3072, nvz=128, scheme=O.FD)
     _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=50, amp=1e-3)
     outs = {}
     for var in ("4", "5"):
@@ -221,3 +222,43 @@ def test_quad_is_the_fd_default_and_times_one_launch(gpu, monkeypatch):
     assert cnt == [5, 0, 0, 0] and ms[0] > 0
     assert "quad" in names[0] and not any(names[1:])
     assert _rel(out, O.step(po, s, 5, moment_form=True, threads=8)) <= 1e-10
+
+
+@pytest.mark.parametrize("chunk", ["", "48"])
+def test_quad_batch_bit_identical_to_solo_runs(gpu, monkeypatch, chunk):
+    """A batch of FD runs of uneven nz and nvz (groups of nvz 64, 128, 256,
+    one with padding bins) takes the four-stage kernel for every group and
+    leaves each run bit-identical to the run advanced alone; chunks of 48
+    rows leave ragged last items."""
+    monkeypatch.delenv("COSENU_BATCH_VARIANT", raising=False)
+    if chunk:
+        monkeypatch.setenv("COSENU_BATCH_ZCHUNK", chunk)
+    specs = [(256, 64), (100, 40), (320, 128), (48, 256), (17, 64), (200, 128)]
+    rng = np.random.default_rng(7)
+    runs, ics = [], []
+    for k, (nz, nvz) in enumerate(specs):
+        p = C.RunParams(nz=nz, nvz=nvz, z0=-nz / 4.0, z1=nz / 4.0, scheme=C.FD,
+                        flags=C.VAC | C.COLL | (C.MAT if k % 2 else 0),
+                        omega=float(rng.uniform(0, 0.5)), theta=float(rng.uniform(1, 40)),
+                        mu=float(rng.uniform(0.5, 1.5)), mode=C.FAST)
+        runs.append(p)
+        ics.append(seeded_ic(p, rng.uniform(0.4, 0.8), rng.uniform(0.4, 0.8),
+                             rng.uniform(0.7, 1.1), seed=200 + k, amp=1e-3))
+    with C.Batch(runs) as b:
+        assert b.info()["fast_variant"] == 5
+        for r, (G, bG, s0) in enumerate(ics):
+            b.set_profile(r, G, bG)
+            b.upload(r, s0)
+        b.step(21)
+        outs = [b.download(r) for r in range(len(runs))]
+    for r, (p, (G, bG, s0)) in enumerate(zip(runs, ics)):
+        with C.Simulation(p) as sim:
+            assert sim.info()["fast_variant"] == 5
+            sim.upload(s0)
+            sim.step(21)
+            solo = sim.download()
+        assert np.array_equal(outs[r], solo), r
+    p, (_, _, s0) = runs[1], ics[1]
+    po = O.RunParams(nz=p.nz, nvz=p.nvz, z0=p.z0, z1=p.z1, scheme=O.FD, flags=p.flags,
+                     omega=p.omega, theta=p.theta, mu=p.mu)
+    assert _rel(outs[1], O.step(po, s0, 21)) <= 1e-10

@@ -4,10 +4,11 @@ The 256-run workload: nz drawn log-uniformly from [4096, 32768] and rounded
 to a multiple of 64, nvz in {64, 128} with equal probability,
 sigma_nu, sigma_nubar ~ U(0.4, 0.8), alpha ~ U(0.7, 1.1), 1e-6 random-phase
 seeds with per-run SplitMix64 seeds; z in [-600, 600], CFL 0.4, mu = 1,
-omega = 0, FV.  (The z range and step count are not fixed by the workload's
-definition; these are the recorded choices.)
+omega = 0, FV (`--scheme fd`: the same runs with the FD scheme).  (The z
+range and step count are not fixed by the workload's definition; these are
+the recorded choices.)
 
-    python -m cosenu_amd.sweep [--runs 256] [--steps 100] [--check 8] [--autotune]
+    python -m cosenu_amd.sweep [--runs 256] [--steps 100] [--check 8] [--autotune] [--scheme fv|fd]
 
 prints one JSON line: aggregate cell-updates/s of the batch and the
 agreement of `--check` randomly chosen runs with the same runs advanced
@@ -25,17 +26,17 @@ import time
 
 import numpy as np
 
-from . import COLL, FAST, FV, STATE, STATE0, VAC, Batch, RunParams, Simulation, batch_autotune
+from . import COLL, FAST, FD, FV, STATE, STATE0, VAC, Batch, RunParams, Simulation, batch_autotune
 from .initial import seeded_ic
 
 
-def sweep_runs(n: int = 256, seed: int = 2026):
+def sweep_runs(n: int = 256, seed: int = 2026, scheme: int = FV):
     rng = np.random.default_rng(seed)
     runs, ics = [], []
     for i in range(n):
         nz = int(round(math.exp(rng.uniform(math.log(4096), math.log(32768))) / 64)) * 64
         nvz = int(rng.choice([64, 128]))
-        p = RunParams(nz=nz, nvz=nvz, z0=-600.0, z1=600.0, CFL=0.4, scheme=FV, flags=VAC | COLL,
+        p = RunParams(nz=nz, nvz=nvz, z0=-600.0, z1=600.0, CFL=0.4, scheme=scheme, flags=VAC | COLL,
                       omega=0.0, theta=1e-6, pmo=1.0, mu=1.0, mode=FAST)
         ics.append((float(rng.uniform(0.4, 0.8)), float(rng.uniform(0.4, 0.8)),
                     float(rng.uniform(0.7, 1.1)), int(seed * 7919 + i)))
@@ -50,9 +51,10 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--check", type=int, default=8)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--scheme", choices=["fv", "fd"], default="fv")
     args = ap.parse_args(argv)
 
-    runs, ics = sweep_runs(args.runs)
+    runs, ics = sweep_runs(args.runs, scheme=FD if args.scheme == "fd" else FV)
     tuned = None
     if args.autotune:
         t0 = time.perf_counter()
@@ -91,7 +93,8 @@ def main(argv=None):
         same += int(np.array_equal(solo, B.download(r)))
     B.close()
     out = {"metric": "batched sweep cell-updates/s", "value": value, "unit": "cell-updates/s",
-           "runs": len(runs), "cells": info["cells"], "work_items": info["work_items"],
+           "runs": len(runs), "scheme": args.scheme, "fast_variant": info["fast_variant"],
+           "cells": info["cells"], "work_items": info["work_items"],
            "groups": info["ngroups"], "z_chunk": info["z_chunk"], "steps": args.steps,
            "ms_per_step": 1e3 * el / args.steps, "setup_s": t_setup,
            "device_GB": info["device_bytes"] / 1e9,

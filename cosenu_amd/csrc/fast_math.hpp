@@ -158,6 +158,38 @@ __device__ inline double wave_sum6(const double m[6], int lane)
     return x;
 }
 
+// Totals of the 6 moments of one ring row over the NW wavefronts of a team:
+// row[ww][0..5] holds wavefront ww's partial sums (slots 6 and 7 unused).
+// One LDS read per lane (lane l reads element l of the row, repeated over
+// the lanes beyond NW * 8), the wavefront partials of a moment summed in a
+// fixed tree by lane moves (row rotate by 8, then row and half swaps:
+// ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7))), after which every row
+// of 16 lanes holds the totals in its lanes 0-5, broadcast to the row by
+// DPP row_newbcast.  One LDS round trip instead of NW * 3 dependent ones.
+template <int NW>
+__device__ inline void moment_totals(const double (*row)[8], int lane, double M[6])
+{
+    static_assert(NW == 2 || NW == 4 || NW == 8, "team of 2, 4 or 8 wavefronts");
+    double x = (&row[0][0])[lane & (NW * 8 - 1)];
+    x += dpp<0x128>(x); // row_ror:8: lanes 0-7 of a row hold w(2r) + w(2r+1)
+    if (NW >= 4) {
+        double y = x;
+        swap16(x, y);
+        x += y;
+    }
+    if (NW >= 8) {
+        double y = x;
+        swap32(x, y);
+        x += y;
+    }
+    M[0] = dpp<0x150>(x); // row_newbcast:m
+    M[1] = dpp<0x151>(x);
+    M[2] = dpp<0x152>(x);
+    M[3] = dpp<0x153>(x);
+    M[4] = dpp<0x154>(x);
+    M[5] = dpp<0x155>(x);
+}
+
 // XCD-aware order: blocks are dealt round-robin to the 8 XCDs, so give each
 // XCD a contiguous run of work (neighbouring chunks share halo rows in that
 // XCD's L2).

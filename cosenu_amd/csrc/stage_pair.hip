@@ -172,14 +172,8 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             K[3] -= hm * r;
         }
         if (coll) {
-            double M[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (int ww = 0; ww < NW; ww++) {
-                const double *slot = ring[rrow & (kRing - 1)][ww];
-#pragma unroll
-                for (int m = 0; m < 6; m++)
-                    M[m] += slot[m];
-            }
+            double M[6];
+            moment_totals<NW>(ring[rrow & (kRing - 1)], lane, M);
             const double S = mudv * (M[0] - vi * M[1]);
             const double R = mudv * (M[2] - vi * M[3]);
             const double D = mudv * (M[4] - vi * M[5]);

@@ -143,14 +143,8 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             K[3] = fma(-hm, r, K[3]);
         }
         if (coll) {
-            double M[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (int ww = 0; ww < NW; ww++) {
-                const double *slot = ring[rrow & (kRing - 1)][ww];
-#pragma unroll
-                for (int m = 0; m < 6; m++)
-                    M[m] += slot[m];
-            }
+            double M[6];
+            moment_totals<NW>(ring[rrow & (kRing - 1)], lane, M);
             const double S = mudv * fma(-vi, M[1], M[0]);
             const double R = mudv * fma(-vi, M[3], M[2]);
             const double D = mudv * fma(-vi, M[5], M[4]);

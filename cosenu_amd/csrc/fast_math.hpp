@@ -160,17 +160,23 @@ __device__ inline double wave_sum6(const double m[6], int lane)
 
 // Totals of the 6 moments of one ring row over the NW wavefronts of a team:
 // row[ww][0..5] holds wavefront ww's partial sums (slots 6 and 7 unused).
-// One LDS read per lane (lane l reads element l of the row, repeated over
-// the lanes beyond NW * 8), the wavefront partials of a moment summed in a
+// One LDS read per lane (moment_partial: lane l reads element l of the row,
+// repeated over the lanes beyond NW * 8; issued early, where the row is
+// known to be complete), the wavefront partials of a moment summed in a
 // fixed tree by lane moves (row rotate by 8, then row and half swaps:
 // ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7))), after which every row
 // of 16 lanes holds the totals in its lanes 0-5, broadcast to the row by
 // DPP row_newbcast.  One LDS round trip instead of NW * 3 dependent ones.
 template <int NW>
-__device__ inline void moment_totals(const double (*row)[8], int lane, double M[6])
+__device__ inline double moment_partial(const double (*row)[8], int lane)
 {
     static_assert(NW == 2 || NW == 4 || NW == 8, "team of 2, 4 or 8 wavefronts");
-    double x = (&row[0][0])[lane & (NW * 8 - 1)];
+    return (&row[0][0])[lane & (NW * 8 - 1)];
+}
+
+template <int NW>
+__device__ inline void moment_totals(double x, double M[6])
+{
     x += dpp<0x128>(x); // row_ror:8: lanes 0-7 of a row hold w(2r) + w(2r+1)
     if (NW >= 4) {
         double y = x;

@@ -141,10 +141,9 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         }
     };
     // right-hand side at the window centre (position 3): z row `row` (wrapped
-    // into [0, nz)), moments in ring slot `rrow % kRing` (rrow unwrapped, as
-    // when they were stored)
-    auto rhs = [&](const double w[4][8], const double F[4], const double Fp[4],
-                   double (*ring)[NW][8], int rrow, int row, double K[4]) {
+    // into [0, nz)), mx = this lane's moment_partial of the row's ring slot
+    auto rhs = [&](const double w[4][8], const double F[4], const double Fp[4], double mx,
+                   int row, double K[4]) {
         const double p = w[0][3], qq = w[1][3], r = w[2][3], s = w[3][3];
 #pragma unroll
         for (int f = 0; f < 4; f++)
@@ -173,7 +172,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         }
         if (coll) {
             double M[6];
-            moment_totals<NW>(ring[rrow & (kRing - 1)], lane, M);
+            moment_totals<NW>(mx, M);
             const double S = mudv * (M[0] - vi * M[1]);
             const double R = mudv * (M[2] - vi * M[3]);
             const double D = mudv * (M[4] - vi * M[5]);
@@ -278,12 +277,16 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
 
         if (((i - ibeg) & (kSync - 1)) == 0) // the same iterations in every team
             __syncthreads();
+        // the two stages' input moments (rows stored 4 iterations ago, behind
+        // a barrier), read before the first stage needs them
+        const double mxa = coll ? moment_partial<NW>(ra[i & (kRing - 1)], lane) : 0.0;
+        const double mxb = coll ? moment_partial<NW>(rb[jb & (kRing - 1)], lane) : 0.0;
 
         // first stage at cell i: its output becomes the newest row of wb
         double oa[4] = {0.0, 0.0, 0.0, 0.0};
         if (cell_a) {
             double K[4];
-            rhs(wa, F, Fa, ra, i, ra_row, K);
+            rhs(wa, F, Fa, mxa, ra_row, K);
 #pragma unroll
             for (int f = 0; f < 4; f++)
                 oa[f] = (PAIR == 0 ? wa[f][3] : ya[f]) + A1.a * K[f];
@@ -305,7 +308,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
                 face(wb, G);
             if (cell_b) {
                 double K[4];
-                rhs(wb, G, Fb, rb, jb, rb_row, K);
+                rhs(wb, G, Fb, mxb, rb_row, K);
                 if (store) {
                     double *o = A2.yout + (size_t)rb_row * zs + loff;
                     double *oq = A1.yout + (size_t)rb_row * zs + loff; // Q (stages 1+2)

@@ -116,13 +116,13 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             ring[row & (kRing - 1)][wit][idx] = t;
     };
     // right-hand side at the window centre: z row `row` (wrapped into [0,
-    // nz)), moments in ring slot `rrow % kRing` (rrow unwrapped, as when they
-    // were stored); the terms in the order of the other FAST kernels.  The
+    // nz)), mx = this lane's moment_partial of the row's ring slot; the terms
+    // in the order of the other FAST kernels.  The
     // window's rows r-2 .. r+2 are in w[f][(RT + k) % 5], k = 0..4.  Every
     // fused multiply-add is written out (this file is built without
     // contraction), so the 5 unrolled copies of the z step evaluate a cell
     // identically whichever copy it falls in.
-    auto rhs = [&](const double (&w)[4][5], auto rot, double (*ring)[NW][8], int rrow, int row,
+    auto rhs = [&](const double (&w)[4][5], auto rot, double mx, int row,
                    double K[4]) __attribute__((always_inline)) {
         constexpr int RT = decltype(rot)::value;
         constexpr int k0 = RT % 5, k1 = (RT + 1) % 5, k2 = (RT + 2) % 5, k3 = (RT + 3) % 5,
@@ -144,7 +144,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         }
         if (coll) {
             double M[6];
-            moment_totals<NW>(ring[rrow & (kRing - 1)], lane, M);
+            moment_totals<NW>(mx, M);
             const double S = mudv * fma(-vi, M[1], M[0]);
             const double R = mudv * fma(-vi, M[3], M[2]);
             const double D = mudv * fma(-vi, M[5], M[4]);
@@ -208,6 +208,15 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         if (T % 2 == 0) // the same iterations in every team
             __syncthreads();
         const int c1 = i, c2 = i - 2, c3 = i - 4, c4 = i - 6;
+        // the stages' input moments (rows stored 2 iterations ago, behind a
+        // barrier), read before stage 1 needs them
+        double mx[4] = {0.0, 0.0, 0.0, 0.0};
+        if (coll) {
+            mx[0] = moment_partial<NW>(rings[0][c1 & (kRing - 1)], lane);
+            mx[1] = moment_partial<NW>(rings[1][c2 & (kRing - 1)], lane);
+            mx[2] = moment_partial<NW>(rings[2][c3 & (kRing - 1)], lane);
+            mx[3] = moment_partial<NW>(rings[3][c4 & (kRing - 1)], lane);
+        }
         // the next window row
         double nn[4];
         load4(A.yin, i + 4, nn);
@@ -218,7 +227,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         double o[4] = {0.0, 0.0, 0.0, 0.0};
         if (c1 >= j0 - 6 && c1 < jend + 6) {
             double K[4];
-            rhs(w0, Rot<T>{}, rings[0], c1, zrow(c1), K);
+            rhs(w0, Rot<T>{}, mx[0], zrow(c1), K);
 #pragma unroll
             for (int f = 0; f < 4; f++)
                 o[f] = fma(ca0, K[f], w0[f][cen0]);
@@ -232,7 +241,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         // window's oldest row (shifted out at the end of the iteration).
         if (c2 >= j0 - 4 && c2 < jend + 4) {
             double K[4];
-            rhs(w1, Rot<T + 1>{}, rings[1], c2, zrow(c2), K);
+            rhs(w1, Rot<T + 1>{}, mx[1], zrow(c2), K);
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 const double y2 = w0[f][old];
@@ -253,7 +262,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         // stage 3 at c3: Y4 = y + dt K3; T = y + (Q + (Y4 - y))/3 replaces Q
         if (c3 >= j0 - 2 && c3 < jend + 2) {
             double K[4];
-            rhs(w2, Rot<T + 1>{}, rings[2], c3, zrow(c3), K);
+            rhs(w2, Rot<T + 1>{}, mx[2], zrow(c3), K);
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 const double yv = yslot(c3, f);
@@ -274,7 +283,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         // stage 4 at c4: y' = T + (dt/6) K4
         if (c4 >= j0 && c4 < jend) {
             double K[4];
-            rhs(w3, Rot<T + 1>{}, rings[3], c4, zrow(c4), K);
+            rhs(w3, Rot<T + 1>{}, mx[3], zrow(c4), K);
             if (store) {
                 double *out = yout + (size_t)zrow(c4) * zs + loff;
 #pragma unroll

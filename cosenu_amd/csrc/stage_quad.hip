@@ -117,11 +117,11 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
     };
     // right-hand side at the window centre: z row `row` (wrapped into [0,
     // nz)), mx = this lane's moment_partial of the row's ring slot; the terms
-    // in the order of the other FAST kernels.  The
-    // window's rows r-2 .. r+2 are in w[f][(RT + k) % 5], k = 0..4.  Every
-    // fused multiply-add is written out (this file is built without
-    // contraction), so the 5 unrolled copies of the z step evaluate a cell
-    // identically whichever copy it falls in.
+    // in the order of the other FAST kernels.  The window's rows r-2 .. r+2
+    // are in w[f][(RT + k) % 5], k = 0..4.  Every fused multiply-add is
+    // written out (this file is built without contraction), so the 5
+    // unrolled copies of the z step evaluate a cell identically whichever
+    // copy it falls in.
     auto rhs = [&](const double (&w)[4][5], auto rot, double mx, int row,
                    double K[4]) __attribute__((always_inline)) {
         constexpr int RT = decltype(rot)::value;

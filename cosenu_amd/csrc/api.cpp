@@ -1100,6 +1100,7 @@ struct Group {
     int nitems = 0;
     bool uniform = false; // every run: nvz == VP, groups of 32 bins of one sign of v
     bool quad = false;    // four stages per launch (FD, 3rd-order dissipation), else two
+    bool any_mat = false; // some run has the matter term
 };
 
 } // namespace
@@ -1163,7 +1164,7 @@ int batch_enqueue_step(cosenu_batch *b, hipStream_t s)
                 if (g.quad) {
                     if (pair == 0)
                         HIP_TRY(launch_stage_quad_batch(runs, g.d_items, g.nitems, g.VP, g.L,
-                                                        g.uniform, s));
+                                                        g.uniform, g.any_mat, s));
                 } else {
                     HIP_TRY(launch_stage_pair_batch(runs, g.d_items, g.nitems, g.VP, g.scheme,
                                                     g.L, pair, g.uniform, s));
@@ -1313,8 +1314,10 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
         b->pairs = b->pairs && pair_supported(g.VP, COSENU_PERIODIC);
     for (Group &g : b->groups) {
         g.quad = b->pairs && bvar != 4;
-        for (int r : g.runs)
+        for (int r : g.runs) {
             g.quad = g.quad && quad_supported(g.VP, COSENU_PERIODIC, runs[r].scheme, runs[r].flags);
+            g.any_mat = g.any_mat || (runs[r].flags & COSENU_MAT) != 0;
+        }
     }
     int rc = 0;
     for (Group &g : b->groups) {

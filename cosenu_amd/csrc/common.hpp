@@ -161,9 +161,10 @@ hipError_t launch_stage_quad(const StageArgs &a, const double ca[4], double *you
 int quad_resident_teams(int VP);
 const char *stage_quad_kernel_name();
 // Batched runs (every run FD with 3rd-order dissipation, periodic, nz >= 16):
-// one launch per step; the new states go to the runs' acc buffers.
+// one launch per step; the new states go to the runs' acc buffers.  mat: some
+// run of the launch has the matter term.
 hipError_t launch_stage_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
-                                   int L, bool uniform, hipStream_t s);
+                                   int L, bool uniform, bool mat, hipStream_t s);
 const char *stage_fast_kernel_name(int VP, int stage, int variant);
 // Batched FAST stage over `nitems` work items (run index, first z cell) of
 // runs with the same VP and scheme; `runs` and `items` are device arrays.

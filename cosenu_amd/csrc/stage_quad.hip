@@ -57,8 +57,10 @@ using Rot = std::integral_constant<int, T>;
 constexpr int kTRing = 5, kYRing = 3; // ring slots per wavefront: Q / T and y
 constexpr int kQRing = (kTRing + kYRing) * 4 * 64; // doubles per wavefront: [slot][field][lane]
 
-// UNI: no padding bins (nvz == VP).  ca[0..3]: dt/2, dt/2, dt, dt/6.
-template <int NW, bool UNI>
+// UNI: no padding bins (nvz == VP).  MAT: the matter term may be on (its
+// potential is then loaded ahead of use, which costs registers that runs
+// without it need not hold).  ca[0..3]: dt/2, dt/2, dt, dt/6.
+template <int NW, bool UNI, bool MAT>
 __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, const double ca1,
                                           const double ca2, const double ca3, double *yout, int L,
                                           int j0, int jend, bool active, double (*rings)[kRing][NW][8],
@@ -96,7 +98,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
 
     const int flags = A.flags;
     const bool coll = (flags & kFlagColl) != 0;
-    const bool vac = (flags & kFlagVac) != 0, mat = (flags & kFlagMat) != 0;
+    const bool vac = (flags & kFlagVac) != 0, mat = MAT && (flags & kFlagMat) != 0;
     const bool advect = !(flags & kFlagAdvecOff);
     // without advection the stencil is multiplied by 0 (no per-value select)
     const double fdfac = advect ? -vi / (12.0 * A.dz) : 0.0;
@@ -115,14 +117,14 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         if ((lane & 7) == 0 && idx < 6)
             ring[row & (kRing - 1)][wit][idx] = t;
     };
-    // right-hand side at the window centre: z row `row` (wrapped into [0,
-    // nz)), mx = this lane's moment_partial of the row's ring slot; the terms
-    // in the order of the other FAST kernels.  The window's rows r-2 .. r+2
+    // right-hand side at the window centre of z row r: mx = this lane's
+    // moment_partial of the row's ring slot, hmr = Hm[r] (MAT); the terms in
+    // the order of the other FAST kernels.  The window's rows r-2 .. r+2
     // are in w[f][(RT + k) % 5], k = 0..4.  Every fused multiply-add is
     // written out (this file is built without contraction), so the 5
     // unrolled copies of the z step evaluate a cell identically whichever
     // copy it falls in.
-    auto rhs = [&](const double (&w)[4][5], auto rot, double mx, int row,
+    auto rhs = [&](const double (&w)[4][5], auto rot, double mx, double hmr,
                    double K[4]) __attribute__((always_inline)) {
         constexpr int RT = decltype(rot)::value;
         constexpr int k0 = RT % 5, k1 = (RT + 1) % 5, k2 = (RT + 2) % 5, k3 = (RT + 3) % 5,
@@ -138,7 +140,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             K[3] = fma(ct2, r, fma(st, p - qq, K[3]));
         }
         if (mat) {
-            const double hm = sig * A.Hm[row];
+            const double hm = sig * hmr;
             K[2] = fma(hm, s, K[2]);
             K[3] = fma(-hm, r, K[3]);
         }
@@ -217,6 +219,14 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             mx[2] = moment_partial<NW>(rings[2][c3 & (kRing - 1)], lane);
             mx[3] = moment_partial<NW>(rings[3][c4 & (kRing - 1)], lane);
         }
+        // the matter potential of the four stages' rows, loaded ahead of use
+        double hmv[4] = {0.0, 0.0, 0.0, 0.0};
+        if (MAT && mat) {
+            hmv[0] = A.Hm[zrow(c1)];
+            hmv[1] = A.Hm[zrow(c2)];
+            hmv[2] = A.Hm[zrow(c3)];
+            hmv[3] = A.Hm[zrow(c4)];
+        }
         // the next window row
         double nn[4];
         load4(A.yin, i + 4, nn);
@@ -227,7 +237,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         double o[4] = {0.0, 0.0, 0.0, 0.0};
         if (c1 >= j0 - 6 && c1 < jend + 6) {
             double K[4];
-            rhs(w0, Rot<T>{}, mx[0], zrow(c1), K);
+            rhs(w0, Rot<T>{}, mx[0], hmv[0], K);
 #pragma unroll
             for (int f = 0; f < 4; f++)
                 o[f] = fma(ca0, K[f], w0[f][cen0]);
@@ -241,7 +251,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         // window's oldest row (shifted out at the end of the iteration).
         if (c2 >= j0 - 4 && c2 < jend + 4) {
             double K[4];
-            rhs(w1, Rot<T + 1>{}, mx[1], zrow(c2), K);
+            rhs(w1, Rot<T + 1>{}, mx[1], hmv[1], K);
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 const double y2 = w0[f][old];
@@ -262,7 +272,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         // stage 3 at c3: Y4 = y + dt K3; T = y + (Q + (Y4 - y))/3 replaces Q
         if (c3 >= j0 - 2 && c3 < jend + 2) {
             double K[4];
-            rhs(w2, Rot<T + 1>{}, mx[2], zrow(c3), K);
+            rhs(w2, Rot<T + 1>{}, mx[2], hmv[2], K);
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 const double yv = yslot(c3, f);
@@ -283,7 +293,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         // stage 4 at c4: y' = T + (dt/6) K4
         if (c4 >= j0 && c4 < jend) {
             double K[4];
-            rhs(w3, Rot<T + 1>{}, mx[3], zrow(c4), K);
+            rhs(w3, Rot<T + 1>{}, mx[3], hmv[3], K);
             if (store) {
                 double *out = yout + (size_t)zrow(c4) * zs + loff;
 #pragma unroll
@@ -316,7 +326,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
     }
 }
 
-template <int NW, bool UNI>
+template <int NW, bool UNI, bool MAT>
 __global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_quad_stage(StageArgs A, double ca0, double ca1, double ca2, double ca3, double *yout,
                        int L, int nchunks, int nteams)
@@ -331,11 +341,12 @@ void cosenu_quad_stage(StageArgs A, double ca0, double ca1, double ca2, double c
         const bool active = g < nteams;
         const int j0 = active ? (int)(((long)g * A.nz) / nteams) : 0;
         const int j1 = active ? (int)(((long)(g + 1) * A.nz) / nteams) : 0;
-        quad_body<NW, UNI>(A, ca0, ca1, ca2, ca3, yout, L, j0, j1, active, rings[team], qr);
+        quad_body<NW, UNI, MAT>(A, ca0, ca1, ca2, ca3, yout, L, j0, j1, active, rings[team], qr);
     } else {
         const bool active = g < nchunks;
         const int j0 = (active ? g : 0) * L;
-        quad_body<NW, UNI>(A, ca0, ca1, ca2, ca3, yout, L, j0, j0 + L, active, rings[team], qr);
+        quad_body<NW, UNI, MAT>(A, ca0, ca1, ca2, ca3, yout, L, j0, j0 + L, active, rings[team],
+                                qr);
     }
 }
 
@@ -343,7 +354,7 @@ void cosenu_quad_stage(StageArgs A, double ca0, double ca1, double ca2, double c
 // item of a launch has the same length L (a run's last item may hold fewer
 // rows), so every team of a block runs the same iterations and barriers.
 // The new state goes to the run's spare buffer (acc).
-template <int NW, bool UNI>
+template <int NW, bool UNI, bool MAT>
 __global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int L)
 {
@@ -357,22 +368,26 @@ void cosenu_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int 
     const BatchRun &r = runs[it.x];
     const StageArgs A = batch_stage_args(r, 1, true);
     const int jend = min(it.y + L, r.nz);
-    quad_body<NW, UNI>(A, r.a[0], r.a[1], r.a[2], r.a[3], r.acc, L, it.y, jend, active,
+    quad_body<NW, UNI, MAT>(A, r.a[0], r.a[1], r.a[2], r.a[3], r.acc, L, it.y, jend, active,
                        rings[team], qring + team * NW * kQRing);
 }
 
 template <int NW>
 hipError_t launch_batch_nw(const BatchRun *runs, const int2 *items, int nitems, int L, bool uni,
-                           hipStream_t s)
+                           bool mat, hipStream_t s)
 {
     constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
     if (nitems <= 0)
         return hipSuccess;
     dim3 grid((nitems + TPB - 1) / TPB), block(64 * NW * TPB);
-    if (uni)
-        hipLaunchKernelGGL((cosenu_quad_batch<NW, true>), grid, block, 0, s, runs, items, nitems, L);
+    if (uni && mat)
+        hipLaunchKernelGGL((cosenu_quad_batch<NW, true, true>), grid, block, 0, s, runs, items, nitems, L);
+    else if (uni)
+        hipLaunchKernelGGL((cosenu_quad_batch<NW, true, false>), grid, block, 0, s, runs, items, nitems, L);
+    else if (mat)
+        hipLaunchKernelGGL((cosenu_quad_batch<NW, false, true>), grid, block, 0, s, runs, items, nitems, L);
     else
-        hipLaunchKernelGGL((cosenu_quad_batch<NW, false>), grid, block, 0, s, runs, items, nitems, L);
+        hipLaunchKernelGGL((cosenu_quad_batch<NW, false, false>), grid, block, 0, s, runs, items, nitems, L);
     return hipGetLastError();
 }
 
@@ -390,12 +405,19 @@ hipError_t launch_nw(const StageArgs &a, const double ca[4], double *yout, int L
         work = nchunks;
     }
     dim3 grid((work + TPB - 1) / TPB), block(64 * NW * TPB);
-    if (uni)
-        hipLaunchKernelGGL((cosenu_quad_stage<NW, true>), grid, block, 0, s, a, ca[0], ca[1], ca[2],
-                           ca[3], yout, L, nchunks, nteams);
-    else
-        hipLaunchKernelGGL((cosenu_quad_stage<NW, false>), grid, block, 0, s, a, ca[0], ca[1],
+    const bool mat = (a.flags & kFlagMat) != 0;
+    if (uni && mat)
+        hipLaunchKernelGGL((cosenu_quad_stage<NW, true, true>), grid, block, 0, s, a, ca[0], ca[1],
                            ca[2], ca[3], yout, L, nchunks, nteams);
+    else if (uni)
+        hipLaunchKernelGGL((cosenu_quad_stage<NW, true, false>), grid, block, 0, s, a, ca[0], ca[1],
+                           ca[2], ca[3], yout, L, nchunks, nteams);
+    else if (mat)
+        hipLaunchKernelGGL((cosenu_quad_stage<NW, false, true>), grid, block, 0, s, a, ca[0], ca[1],
+                           ca[2], ca[3], yout, L, nchunks, nteams);
+    else
+        hipLaunchKernelGGL((cosenu_quad_stage<NW, false, false>), grid, block, 0, s, a, ca[0],
+                           ca[1], ca[2], ca[3], yout, L, nchunks, nteams);
     return hipGetLastError();
 }
 
@@ -408,7 +430,7 @@ int resident_teams_nw()
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void *>(&cosenu_quad_stage<NW, false>), 64 * NW * TPB, 0) !=
+            &nb, reinterpret_cast<const void *>(&cosenu_quad_stage<NW, false, true>), 64 * NW * TPB, 0) !=
         hipSuccess)
         return 0;
     return nb * prop.multiProcessorCount * TPB;
@@ -443,14 +465,14 @@ hipError_t launch_stage_quad(const StageArgs &a, const double ca[4], double *you
 }
 
 hipError_t launch_stage_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
-                                   int L, bool uniform, hipStream_t s)
+                                   int L, bool uniform, bool mat, hipStream_t s)
 {
     if (L < 1)
         return hipErrorInvalidValue;
     switch (VP) {
-    case 64: return launch_batch_nw<2>(runs, items, nitems, L, uniform, s);
-    case 128: return launch_batch_nw<4>(runs, items, nitems, L, uniform, s);
-    case 256: return launch_batch_nw<8>(runs, items, nitems, L, uniform, s);
+    case 64: return launch_batch_nw<2>(runs, items, nitems, L, uniform, mat, s);
+    case 128: return launch_batch_nw<4>(runs, items, nitems, L, uniform, mat, s);
+    case 256: return launch_batch_nw<8>(runs, items, nitems, L, uniform, mat, s);
     default: return hipErrorInvalidValue;
     }
 }

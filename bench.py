@@ -53,20 +53,19 @@ STAGE_BYTES = {1: (192, 320, 320, 256), 2: (128, 192, 192, 320), 3: (132, 196, 1
                4: (192, 256, 0, 0), 5: (128, 0, 0, 0)}
 
 
-def cpu_baseline(scheme: str = "fv", seconds_target: float = 12.0):
+def cpu_baseline(scheme: str = "fv", nvz: int = 128, dz: float = 3000.0 / 65536,
+                 seconds_target: float = 12.0):
     """Time the reference CPU code (oracle/_ref, built from the reference's own
     headers) or, where that build is absent, the oracle restatement, on a
-    z-subsampled slice of the workload: same scheme and dz, nvz = 128,
-    nz = 2048."""
+    z-subsampled slice of the workload: same scheme, dz and nvz, nz = 2048."""
     import numpy as np
     import oracle as O
     from cosenu_amd.initial import seeded_ic
 
     threads = min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    dz = 3000.0 / 65536
     nz = 2048
-    p = O.RunParams(nz=nz, nvz=128, z0=-nz * dz / 2, z1=nz * dz / 2, flags=O.VAC | O.COLL,
+    p = O.RunParams(nz=nz, nvz=nvz, z0=-nz * dz / 2, z1=nz * dz / 2, flags=O.VAC | O.COLL,
                     omega=0.0, theta=1e-6, scheme=O.FD if scheme == "fd" else O.FV)
     G, bG, s = seeded_ic(p, 0.6, 0.53, 0.92, seed=1)
     use_ref = O.ref_available(O.ref_variant(p))
@@ -97,10 +96,10 @@ def cpu_baseline(scheme: str = "fv", seconds_target: float = 12.0):
         os.dup2(saved, 1)
         os.close(saved)
         os.close(devnull)
-    cells = nz * 128
+    cells = nz * nvz
     return {"value": cells * n / el, "unit": "cell-updates/s", "cores": threads, "kind": kind,
             "sample": f"{n} RK4 steps of {'FD' if scheme == 'fd' else 'FV-WENO7'} (vac+coll) on "
-                      f"nz={nz} x nvz=128, same dz as the workload, {threads} OpenMP threads, "
+                      f"nz={nz} x nvz={nvz}, same dz as the workload, {threads} OpenMP threads, "
                       f"{el:.1f} s"}
 
 
@@ -111,7 +110,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--scheme", choices=["fv", "fd"], default="fv")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c4"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c4", "wide"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path "
@@ -256,7 +255,7 @@ def main():
     }
     sim.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.scheme)
+        out["cpu_baseline"] = cpu_baseline(args.scheme, p.nvz, (p.z1 - p.z0) / p.nz)
     print(json.dumps(out), flush=True)
 
 

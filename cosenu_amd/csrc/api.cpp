@@ -212,8 +212,11 @@ int validate(const cosenu_params *p)
     if (!(p->z1 > p->z0) || !(p->v1 > p->v0) || !(p->CFL > 0))
         return fail(COSENU_EINVAL, "need z1 > z0, v1 > v0, CFL > 0");
     const int VP = ((p->nvz + 63) / 64) * 64;
-    if (p->mode == COSENU_FAST && !fast_supported(VP, p->bc))
-        return fail(COSENU_ENOTSUP, "FAST mode supports nvz up to 256");
+    // nvz > 256: the barrier-free kernel alone (periodic; the groups of 32
+    // bins are checked for one sign of v when the context is created)
+    if (p->mode == COSENU_FAST && !fast_supported(VP, p->bc) &&
+        !(VP > 256 && p->bc == COSENU_PERIODIC))
+        return fail(COSENU_ENOTSUP, "FAST mode with nvz > 256 needs periodic boundaries");
     return 0;
 }
 
@@ -564,6 +567,11 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
     c->uniform = stream_supported(c->nvz, c->VP, COSENU_PERIODIC, vz.data()) != 0 &&
                  env_int("COSENU_UNIFORM", 1) != 0;
     c->stream_ok = p->mode == COSENU_FAST && c->uniform && p->bc == COSENU_PERIODIC;
+    if (p->mode == COSENU_FAST && c->VP > 256 && !c->stream_ok) {
+        delete c;
+        return fail(COSENU_ENOTSUP, "FAST mode with nvz > 256 needs nvz a multiple of 64 and "
+                                    "every group of 32 velocity bins on one side of v = 0");
+    }
     if (p->mode == COSENU_FAST) {
         // z_chunk < 0 in the tune table = contiguous-range mode with that many
         // times the resident teams
@@ -579,6 +587,11 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             // against 28 us for four-stage ranges)
             c->variant = 3;
             chunk = c->nz % 2 ? 1 : 2;
+        } else if (c->VP > 256) {
+            // the team kernels hold a team of VP/32 wavefronts in one block
+            // of at most 8: wider grids take the barrier-free kernel
+            c->variant = 3;
+            chunk = -1;
         } else {
             // FD with 3rd-order dissipation: four stages per launch (nz =
             // 65536, nvz = 128: 0.54 ms per step against 0.84 ms for two-stage
@@ -597,6 +610,8 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
         }
         if (ez != 0)
             chunk = ez;
+        if (c->VP > 256)
+            c->variant = 3; // the only kernel for nvz > 256 (stream_ok holds)
         if (c->variant == 3 && !c->stream_ok)
             c->variant = 2; // shapes the barrier-free kernel does not take
         if (c->variant == 5 && (!quad_supported(c->VP, p->bc, p->scheme, p->flags) || c->nz < 16))
@@ -1025,7 +1040,7 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     for (int var = q.bc == COSENU_OPEN ? 2 : 1; var <= 5 && !rc; var++)
         for (int L : cands) {
             if (rc || (L > 0 && (L > c->nz || c->nz % L)) || (L < 0 && var == 1) ||
-                (var == 3 && !c->stream_ok) ||
+                (var == 3 && !c->stream_ok) || (var != 3 && c->VP > 256) ||
                 (var == 4 && (!pair_supported(c->VP, q.bc) || c->nz < 16)) ||
                 (var == 5 && (!quad_supported(c->VP, q.bc, q.scheme, q.flags) || c->nz < 16)))
                 continue;
@@ -1262,6 +1277,8 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
             return fail(COSENU_EINVAL, "all runs of a batch must use one device");
         if (runs[r].bc != COSENU_PERIODIC)
             return fail(COSENU_ENOTSUP, "run " + std::to_string(r) + ": batches take periodic runs");
+        if (runs[r].nvz > 256)
+            return fail(COSENU_ENOTSUP, "run " + std::to_string(r) + ": batches take nvz up to 256");
     }
     HIP_TRY(hipSetDevice(runs[0].device));
     cosenu_batch *b = new cosenu_batch();

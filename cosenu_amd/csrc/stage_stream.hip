@@ -127,8 +127,17 @@ void cosenu_stream_stage(StageArgs A, int L, int nwork, int nteams)
             o[f] = rp[off[f]];
     };
     // Partial moments of a row: lane gg*8 + i holds moment i of group gg.
+    // More than 8 groups (VP > 256): lane gg*8 + i holds the sum of moment i
+    // over groups gg, gg + 8, gg + 16, ... in that order.
     auto load_mom = [&](int r) -> double {
-        return lane < G * kMomSlot ? A.mom_in[(size_t)r * G * kMomSlot + lane] : 0.0;
+        const double *row = A.mom_in + (size_t)r * G * kMomSlot;
+        if (G <= 8)
+            return lane < G * kMomSlot ? row[lane] : 0.0;
+        double t = row[lane];
+        for (int c = 64; c < G * kMomSlot; c += 64)
+            if (c + lane < G * kMomSlot) // G need not be a multiple of 8
+                t += row[c + lane];
+        return t;
     };
     // Totals over the groups (fixed order), moment i in lane i.
     auto sum_mom = [&](double t) -> double {

@@ -144,6 +144,9 @@ def workload(name: str, mode: int = FAST, scheme: int = FV) -> Workload:
     if name == "c1":   # nz=65536 on [-1500,1500], nvz=128, 20000 steps, FV and FD
         return Workload(name, RunParams(nz=65536, nvz=128, z0=-1500, z1=1500, **base), 20000,
                         0.6, 0.53, 0.92, gaussian_seed=True)
+    if name == "wide":  # nz=16384 on [-375,375] (c1's dz), nvz=512, c1's physics
+        return Workload(name, RunParams(nz=16384, nvz=512, z0=-375, z1=375, **base), 20000,
+                        0.6, 0.53, 0.92, gaussian_seed=True)
     if name == "c4":   # nz=2^21 on [-20000,20000], nvz=256, 5000 steps
         return Workload(name, RunParams(nz=2 ** 21, nvz=256, z0=-20000, z1=20000, **base), 5000,
                         0.6, 0.5, 0.9)

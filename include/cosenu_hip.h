@@ -87,7 +87,11 @@ typedef struct cosenu_info {
                             launch (default otherwise), 3 = barrier-free
                             (default for small problems), 2 = one stage per
                             launch, two lanes per bin (nu / nubar halves), 1 =
-                            one lane per bin; 0 in STRICT */
+                            one lane per bin; 0 in STRICT.  nvz > 256: 3 only
+                            (periodic, nvz a multiple of 64 with every group
+                            of 32 bins on one side of v = 0; otherwise
+                            cosenu_create returns COSENU_ENOTSUP and such grids
+                            run in STRICT mode) */
     int range_teams;     /* FAST mode: > 0 when each of this many teams streams
                             one contiguous z range (z_chunk = its length) */
 } cosenu_info;

@@ -1,0 +1,94 @@
+"""FAST mode for velocity grids wider than 256 bins (nvz = 384 .. 1024).
+
+The team kernels keep the VP/32 wavefronts of a z range in one block of at
+most 8; wider grids take the barrier-free kernel (variant 3), in which each
+wavefront owns 32 bins and the per-z moments pass through memory as one
+partial per (z row, group of 32 bins).  Beyond 8 groups a lane first adds
+the partials of groups g, g + 8, g + 16, ... and the fixed lane-move tree
+then adds the 8 results.  The reference runs any nvz (rhs_fv.hpp:188-233
+loops over all bins); these tests pin the wide path against the oracle and
+against STRICT mode, which is bit-identical to the reference's arithmetic.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import cosenu_amd as C
+from cosenu_amd.initial import seeded_ic
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(scheme, nz, nvz, flags=O.VAC | O.COLL, **kw):
+    args = dict(nz=nz, nvz=nvz, z0=-nz / 4.0, z1=nz / 4.0, scheme=scheme, flags=flags,
+                omega=0.3, theta=37.0, pmo=1.0, mu=1.0, **kw)
+    return O.RunParams(**args), C.RunParams(mode=C.FAST, **args)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("nvz", [384, 512, 1024])
+@pytest.mark.parametrize("scheme,flags", [(O.FV, O.VAC | O.COLL), (O.FD, O.VAC | O.COLL),
+                                          (O.FV, O.VAC | O.MAT | O.COLL), (O.FD, O.COLL | O.KO5)])
+def test_wide_rk4_matches_oracle(gpu, scheme, flags, nvz):
+    nz = 64 if nvz < 1024 else 32
+    po, pc = _pair(scheme, nz, nvz, flags)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=41, amp=1e-3)
+    ref = O.step(po, s, 10)
+    with C.Simulation(pc) as sim:
+        assert sim.info()["fast_variant"] == 3
+        sim.upload(s)
+        sim.step(10)
+        out = sim.download()
+    print(f"nvz={nvz} scheme={scheme} flags={flags}: rel {_rel(out, ref):.3e}", flush=True)
+    assert _rel(out, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("nvz", [512, 1024])
+def test_wide_rhs_matches_oracle(gpu, nvz):
+    po, pc = _pair(O.FV, 32, nvz)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=42, amp=1e-3)
+    ref = O.rhs(po, s)
+    with C.Simulation(pc) as sim:
+        sim.upload(s)
+        out = sim.rhs()
+    assert _rel(out, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("scheme", [O.FV, O.FD])
+def test_wide_fast_agrees_with_strict(gpu, scheme):
+    """Longer run, flavour-unstable regime: FAST (moment form, lanes reduced
+    across 16 groups) against STRICT (the reference's per-cell loop)."""
+    po, pc = _pair(scheme, 256, 512)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=43, amp=1e-3)
+    outs = {}
+    for mode in (C.STRICT, C.FAST):
+        pc.mode = mode
+        with C.Simulation(pc) as sim:
+            sim.upload(s)
+            sim.step(60)
+            outs[mode] = sim.download()
+    assert np.isfinite(outs[C.FAST]).all()
+    assert _rel(outs[C.FAST], outs[C.STRICT]) <= 1e-10
+
+
+def test_wide_diagnostics_conserve_trace(gpu):
+    po, pc = _pair(O.FV, 128, 512, flags=O.VAC | O.COLL)
+    G, bG, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=44, amp=1e-3)
+    with C.Simulation(pc) as sim:
+        sim.set_profile(G, bG)
+        sim.upload(s, C.STATE0)
+        sim.upload(s)
+        d0 = sim.diagnostics()
+        sim.step(40)
+        d1 = sim.diagnostics()
+    assert d1["nonfinite"] == 0
+    assert abs(d1["trace"] - d0["trace"]) <= 1e-12 * abs(d0["trace"])
+    assert abs(d1["btrace"] - d0["btrace"]) <= 1e-12 * abs(d0["btrace"])
+
+
+def test_wide_autotune_keeps_the_barrier_free_kernel(gpu):
+    variant, chunk, ms = C.autotune(C.RunParams(nz=256, nvz=512, mode=C.FAST))
+    assert variant == 3 and ms > 0

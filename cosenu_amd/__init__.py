@@ -137,6 +137,12 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
 
 
+def _need(ok: bool, what: str):
+    """Shape checks before raw pointers reach the library (kept under -O)."""
+    if not ok:
+        raise ValueError(what)
+
+
 @dataclass
 class RunParams:
     """A run's grid and physics: the keys of the reference's job.config
@@ -217,8 +223,8 @@ class Simulation:
 
     # -- data in/out ---------------------------------------------------------
     def _planes(self, arr: np.ndarray):
-        assert arr.shape == (8, self.p.nvz, self.p.nz) and arr.dtype == np.float64
-        assert arr.flags.c_contiguous
+        _need(arr.shape == (8, self.p.nvz, self.p.nz) and arr.dtype == np.float64
+              and arr.flags.c_contiguous, "state must be C-contiguous float64 [8, nvz, nz]")
         return _planes_t(*[arr[f].ctypes.data_as(_dp) for f in range(8)])
 
     def set_profile(self, G: np.ndarray, bG: np.ndarray):
@@ -240,7 +246,7 @@ class Simulation:
 
     def upload_interleaved(self, buf: np.ndarray, which: int = STATE):
         b = np.ascontiguousarray(buf, dtype=np.float64)
-        assert b.size == 8 * self.p.nvz * self.p.nz
+        _need(b.size == 8 * self.p.nvz * self.p.nz, "interleaved state must hold 8*nvz*nz values")
         _check(lib().cosenu_upload_interleaved(self._h, which, _ptr(b)), "upload_interleaved")
 
     def download_interleaved(self, which: int = STATE) -> np.ndarray:
@@ -280,20 +286,22 @@ class Simulation:
         """state[f, v, z] = a[f, v] * b[f, z] built on the device."""
         a = np.ascontiguousarray(a, dtype=np.float64)
         b = np.ascontiguousarray(b, dtype=np.float64)
-        assert a.shape == (8, self.p.nvz) and b.shape == (8, self.p.nz)
+        _need(a.shape == (8, self.p.nvz) and b.shape == (8, self.p.nz),
+              "separable factors must be [8, nvz] and [8, nz]")
         _check(lib().cosenu_init_separable(self._h, which, _ptr(a), _ptr(b)), "init_separable")
 
     def set_profile_v(self, G: np.ndarray, bG: np.ndarray):
         G = np.ascontiguousarray(G, dtype=np.float64)
         bG = np.ascontiguousarray(bG, dtype=np.float64)
-        assert G.shape == (self.p.nvz,) and bG.shape == (self.p.nvz,)
+        _need(G.shape == (self.p.nvz,) and bG.shape == (self.p.nvz,),
+              "profiles must have nvz values each")
         _check(lib().cosenu_set_profile_v(self._h, _ptr(G), _ptr(bG)), "set_profile_v")
 
     def copy_state(self, src: int, dst: int):
         _check(lib().cosenu_copy_state(self._h, src, dst), "cosenu_copy_state")
 
     def snapshot_pee_async(self, buf: "PinnedBuffer"):
-        assert buf.n >= self.p.nvz * self.p.nz
+        _need(buf.n >= self.p.nvz * self.p.nz, "snapshot buffer must hold nvz*nz values")
         _check(lib().cosenu_snapshot_pee_async(self._h, buf.ptr), "snapshot_pee_async")
 
     def snapshot_wait(self):
@@ -368,7 +376,8 @@ class Batch:
 
     def _planes(self, r: int, arr: np.ndarray):
         p = self.runs[r]
-        assert arr.shape == (8, p.nvz, p.nz) and arr.dtype == np.float64 and arr.flags.c_contiguous
+        _need(arr.shape == (8, p.nvz, p.nz) and arr.dtype == np.float64 and arr.flags.c_contiguous,
+              "state must be C-contiguous float64 [8, nvz, nz]")
         return _planes_t(*[arr[f].ctypes.data_as(_dp) for f in range(8)])
 
     def set_profile(self, r: int, G, bG):

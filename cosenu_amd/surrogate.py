@@ -102,7 +102,8 @@ class MLP:
 
     def set_params(self, flat):
         flat = _f32(flat)
-        assert flat.size == self.nparams
+        if flat.size != self.nparams:
+            raise ValueError(f"expected {self.nparams} parameters, got {flat.size}")
         self._chk(_mlib().cosenu_mlp_set_params(self._h, _p(flat)), "set_params")
 
     def get_params(self) -> np.ndarray:

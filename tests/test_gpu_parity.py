@@ -259,6 +259,11 @@ def test_errors_are_reported_not_fatal(gpu):
     with C.Simulation(C.RunParams(nz=64, nvz=16, mode=C.STRICT)) as sim:
         with pytest.raises(C.CosenuError, match="profile"):
             sim.diagnostics()
+        # host buffers of the wrong shape never reach the library
+        with pytest.raises(ValueError):
+            sim.upload(np.zeros((8, 16, 63)))
+        with pytest.raises(ValueError):
+            sim.set_profile_v(np.ones(15), np.ones(16))
 
 
 def test_separable_initial_state_equals_host_state(gpu):

@@ -53,3 +53,20 @@ def test_library_loads_without_a_gpu():
     text = open(HEADER).read()
     m = re.search(r"#define\s+COSENU_ABI_VERSION\s+(\d+)", text)
     assert m and L.cosenu_abi_version() == int(m.group(1))
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+@pytest.mark.parametrize("kw,match", [
+    (dict(nz=64, nvz=0), "nvz"),
+    (dict(nz=64, nvz=2048), "nvz"),
+    (dict(nz=4, nvz=64), "nz"),
+    (dict(nz=64, nvz=512, bc=1), "FAST mode with nvz > 256 needs periodic"),
+    (dict(nz=64, nvz=64, scheme=0, flags=16 | 2), "KO5"),
+])
+def test_invalid_parameters_are_returned_before_any_device_call(kw, match):
+    """cosenu_create validates its parameters first and reports a status and
+    a message (the reference exits the process, parser.hpp:220-243); no HIP
+    call is made, so this runs without a GPU."""
+    import cosenu_amd as C
+    with pytest.raises(C.CosenuError, match=match):
+        C.Simulation(C.RunParams(mode=C.FAST, **kw))

@@ -90,6 +90,7 @@ def lib() -> ctypes.CDLL:
     L.cosenu_upload_interleaved.argtypes = [vp, ctypes.c_int, _dp]
     L.cosenu_download_interleaved.argtypes = [vp, ctypes.c_int, _dp]
     L.cosenu_step.argtypes = [vp, ctypes.c_int]
+    L.cosenu_prepare.argtypes = [vp]
     L.cosenu_sync.argtypes = [vp]
     L.cosenu_rhs.argtypes = [vp, _planes_t, ctypes.c_ssize_t, ctypes.c_int]
     L.cosenu_diagnostics.argtypes = [vp, ctypes.POINTER(_Diag)]
@@ -257,6 +258,10 @@ class Simulation:
     # -- compute ---------------------------------------------------------------
     def step(self, nsteps: int = 1):
         _check(lib().cosenu_step(self._h, int(nsteps)), "cosenu_step")
+
+    def prepare(self):
+        """Capture the step graph now, so the next long step() pays no capture."""
+        _check(lib().cosenu_prepare(self._h), "cosenu_prepare")
 
     def sync(self):
         _check(lib().cosenu_sync(self._h), "cosenu_sync")

@@ -211,7 +211,7 @@ int validate(const cosenu_params *p)
         return fail(COSENU_EINVAL, "mode must be COSENU_STRICT or COSENU_FAST");
     if (!(p->z1 > p->z0) || !(p->v1 > p->v0) || !(p->CFL > 0))
         return fail(COSENU_EINVAL, "need z1 > z0, v1 > v0, CFL > 0");
-    const int VP = ((p->nvz + 63) / 64) * 64;
+    const int VP = padded_bins(p->nvz);
     // nvz > 256: the barrier-free kernel alone (periodic; the groups of 32
     // bins are checked for one sign of v when the context is created)
     if (p->mode == COSENU_FAST && !fast_supported(VP, p->bc) &&
@@ -265,7 +265,9 @@ void tune_load_locked()
         std::istringstream ss(line);
         TuneKey k;
         TuneVal v;
-        if (ss >> k.VP >> k.scheme >> k.nz >> v.variant >> v.chunk >> v.ms)
+        // entries of kernels this build does not have are skipped
+        if (ss >> k.VP >> k.scheme >> k.nz >> v.variant >> v.chunk >> v.ms && v.variant >= 2 &&
+            v.variant <= 5)
             g_tune[k] = v;
     }
 }
@@ -417,8 +419,8 @@ StageArgs stage_args(const cosenu_ctx *c, int stage)
     r.a[0] = 0.5 * c->d.dt; r.a[1] = 0.5 * c->d.dt; r.a[2] = c->d.dt; r.a[3] = 1.0 / 6.0 * c->d.dt;
     r.dz = c->d.dz; r.dv = c->d.dv; r.ct = c->d.ct; r.st = c->d.st;
     r.pmo = c->p.pmo; r.mu = c->p.mu; r.ko = c->p.ko;
-    // the split-lane and barrier-free FAST kernels use the stage-value form
-    StageArgs a = batch_stage_args(r, stage, c->p.mode == COSENU_FAST && c->variant >= 2);
+    // the FAST kernels use the stage-value form
+    StageArgs a = batch_stage_args(r, stage, c->p.mode == COSENU_FAST);
     if (c->p.mode == COSENU_FAST && c->variant == 3) {
         // moments travel with the stage states: y, A (Y2), B (Y3), acc (Y4)
         a.mom_in = c->mom[stage - 1];
@@ -444,7 +446,7 @@ hipError_t launch_stage(const cosenu_ctx *c, int stage, hipStream_t s)
     if (c->p.mode == COSENU_FAST && c->variant == 3)
         return launch_stage_stream(a, stage, c->z_chunk, c->nteams, s);
     if (c->p.mode == COSENU_FAST)
-        return launch_stage_fast(a, stage, c->z_chunk, c->variant, c->nteams, s);
+        return launch_stage_fast(a, stage, c->z_chunk, c->nteams, s);
     return launch_stage_strict(a, stage, s);
 }
 
@@ -540,6 +542,33 @@ int capture_graph(hipStream_t s, int nsteps, Launch launch_step, hipGraphExec_t 
     return 0;
 }
 
+// Steps per captured graph.  Two- and four-stage launches alternate the
+// state buffers every step: a graph of an even number of steps ends on the
+// buffer it started from.
+int graph_steps(const cosenu_ctx *c)
+{
+    int gsteps = env_int("COSENU_GRAPH_STEPS", 16);
+    if (swaps_state(c))
+        gsteps &= ~1;
+    return gsteps;
+}
+
+// The graph of gsteps steps starting from the current state buffer.
+int ensure_graph(cosenu_ctx *c, int gsteps)
+{
+    if (c->graph && c->graph_steps == gsteps && c->graph_y == c->y)
+        return 0;
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    c->graph_y = c->y;
+    auto one = [c]() -> int { return enqueue_step(c, c->stream); };
+    RC_TRY(capture_graph(c->stream, gsteps, one, &c->graph));
+    c->graph_steps = gsteps;
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -560,7 +589,7 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
     c->d = derive(*p);
     c->nz = p->nz;
     c->nvz = p->nvz;
-    c->VP = ((p->nvz + 63) / 64) * 64;
+    c->VP = padded_bins(p->nvz);
     const size_t n_state = (size_t)c->nz * 8 * c->VP;
     std::vector<double> vz, Hm;
     fill_grids(*p, c->d, c->VP, vz, Hm);
@@ -604,7 +633,7 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             chunk = -1;
         }
         int ev = env_int("COSENU_FAST_VARIANT", 0), ez = env_int("COSENU_ZCHUNK", 0);
-        if (ev >= 1 && ev <= 5 && ev != c->variant) {
+        if (ev >= 2 && ev <= 5 && ev != c->variant) {
             c->variant = ev;
             chunk = -1; // that variant's default launch: contiguous ranges
         }
@@ -620,7 +649,7 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             c->variant = 2;
         if (p->bc == COSENU_OPEN)
             c->variant = 2; // the open boundary exists in the split-lane kernel only
-        if (chunk < 0 && c->variant >= 2) {
+        if (chunk < 0) {
             c->nteams = range_teams(c->nz, c->VP, p->scheme, p->bc, c->variant, -chunk);
             c->z_chunk = (c->nz + c->nteams - 1) / c->nteams;
         } else {
@@ -830,6 +859,17 @@ int cosenu_copy_state(cosenu_ctx *c, int from, int to)
     return 0;
 }
 
+int cosenu_prepare(cosenu_ctx *c)
+{
+    if (!c)
+        return fail(COSENU_EINVAL, "cosenu_prepare: null context");
+    HIP_TRY(hipSetDevice(c->p.device));
+    const int gsteps = graph_steps(c);
+    if (c->timing || gsteps <= 0)
+        return 0;
+    return ensure_graph(c, gsteps);
+}
+
 int cosenu_step(cosenu_ctx *c, int nsteps)
 {
     if (!c || nsteps < 0)
@@ -839,23 +879,10 @@ int cosenu_step(cosenu_ctx *c, int nsteps)
         // the state may have been replaced since the last step: its moments
         // are rebuilt here, after that every stage 4 leaves them current
         HIP_TRY(launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->VP, c->stream));
-    // two- and four-stage launches alternate the state buffers every step: a
-    // graph of an even number of steps ends on the buffer it started from
-    int gsteps = env_int("COSENU_GRAPH_STEPS", 16);
-    if (swaps_state(c))
-        gsteps &= ~1;
+    const int gsteps = graph_steps(c);
     int done = 0;
     if (!c->timing && gsteps > 0 && nsteps >= gsteps) {
-        if (!c->graph || c->graph_steps != gsteps || c->graph_y != c->y) {
-            if (c->graph) {
-                (void)hipGraphExecDestroy(c->graph);
-                c->graph = nullptr;
-            }
-            c->graph_y = c->y;
-            auto one = [c]() -> int { return enqueue_step(c, c->stream); };
-            RC_TRY(capture_graph(c->stream, gsteps, one, &c->graph));
-            c->graph_steps = gsteps;
-        }
+        RC_TRY(ensure_graph(c, gsteps));
         for (; done + gsteps <= nsteps; done += gsteps)
             HIP_TRY(hipGraphLaunch(c->graph, c->stream));
     }
@@ -895,8 +922,7 @@ int cosenu_rhs(cosenu_ctx *c, double *const out[8], ptrdiff_t stride, int off)
         a.mom_out = nullptr;
         HIP_TRY(launch_stage_stream(a, 0, c->z_chunk, c->nteams, c->stream));
     } else if (c->p.mode == COSENU_FAST) { // variants 4, 5 evaluate single RHSs with variant 2
-        HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->variant >= 4 ? 2 : c->variant, c->nteams,
-                                c->stream));
+        HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->nteams, c->stream));
     } else {
         HIP_TRY(launch_rhs_strict(r, c->stream));
     }
@@ -1009,7 +1035,7 @@ const char *cosenu_stage_kernel_name(cosenu_ctx *c, int stage)
         return stage == 0 ? stage_quad_kernel_name() : "";
     if (two_stage_launches(c))
         return stage < 2 ? stage_pair_kernel_name(stage) : "";
-    return c->p.mode == COSENU_FAST ? stage_fast_kernel_name(c->VP, stage, c->variant)
+    return c->p.mode == COSENU_FAST ? stage_fast_kernel_name(c->VP, stage)
                                     : stage_strict_kernel_name(stage);
 }
 
@@ -1037,9 +1063,9 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     // chunk lengths, then (variants 2-5) contiguous ranges with 1 or 2 times
     // the resident teams, encoded as chunk = -1, -2
     const int cands[] = {1, 2, 4, 8, 16, 32, 64, 128, 256, -1, -2};
-    for (int var = q.bc == COSENU_OPEN ? 2 : 1; var <= 5 && !rc; var++)
+    for (int var = 2; var <= 5 && !rc; var++)
         for (int L : cands) {
-            if (rc || (L > 0 && (L > c->nz || c->nz % L)) || (L < 0 && var == 1) ||
+            if (rc || (L > 0 && (L > c->nz || c->nz % L)) ||
                 (var == 3 && !c->stream_ok) || (var != 3 && c->VP > 256) ||
                 (var == 4 && (!pair_supported(c->VP, q.bc) || c->nz < 16)) ||
                 (var == 5 && (!quad_supported(c->VP, q.bc, q.scheme, q.flags) || c->nz < 16)))
@@ -1087,7 +1113,7 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     cosenu_destroy(c);
     if (rc)
         return rc;
-    tune_store(((q.nvz + 63) / 64) * 64, q.scheme, q.nz, {bv, bl, best}); // ms per step
+    tune_store(padded_bins(q.nvz), q.scheme, q.nz, {bv, bl, best}); // ms per step
     if (variant)
         *variant = bv;
     if (z_chunk)
@@ -1299,7 +1325,7 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
     for (int r = 0; r < nruns; r++) {
         b->P[r].mode = COSENU_FAST;
         b->D.push_back(derive(b->P[r]));
-        const int VP = ((runs[r].nvz + 63) / 64) * 64;
+        const int VP = padded_bins(runs[r].nvz);
         auto key = std::make_pair(VP, runs[r].scheme);
         if (!gid.count(key)) {
             gid[key] = (int)b->groups.size();

@@ -1,8 +1,9 @@
 // Shared definitions of the COSEnu MI355X kernels.
 //
 // Device state layout ("row layout"): for every z cell one row of 8 fields,
-// each field a run of VP doubles over the velocity bins, VP = nvz rounded up
-// to a multiple of 64 (one wavefront of lanes).  Element (z, f, v) lives at
+// each field a run of VP doubles over the velocity bins, VP = padded_bins(nvz)
+// (64, 128 or 256 up to 256 bins, else nvz rounded up to a multiple of 64).
+// Element (z, f, v) lives at
 //     ((size_t)z * 8 + f) * VP + v.
 // A 64-lane load of one field at one z is one contiguous 512-byte segment.
 // Bins v >= nvz are padding, kept at zero and never written by the stage
@@ -21,14 +22,30 @@ constexpr int kFlagVac = 1, kFlagMat = 2, kFlagColl = 4, kFlagAdvecOff = 8, kFla
 
 __host__ __device__ inline size_t row_index(int z, int f, int VP) { return ((size_t)z * 8 + f) * VP; }
 
+// Bins per field row on the device.  Up to 256 bins the row is 64, 128 or
+// 256 wide, the widths every FAST kernel is built for (a team holds VP/32
+// wavefronts), so nvz = 129..256 takes 256 with zero padding bins; beyond
+// 256 the row is nvz rounded up to a multiple of 64 (the barrier-free
+// kernel, one wavefront per 64 bins).
+inline int padded_bins(int nvz)
+{
+    if (nvz <= 64)
+        return 64;
+    if (nvz <= 128)
+        return 128;
+    if (nvz <= 256)
+        return 256;
+    return ((nvz + 63) / 64) * 64;
+}
+
 // Everything a stage kernel needs.  Stage s in 1..4 evaluates K = RHS(yin).
 // Two forms of the RK4 combination (nuosc.hpp:320-348):
-//  classic (STRICT, and the one-lane FAST variant), the reference's algebra:
+//  classic (STRICT), the reference's algebra:
 //   s=1: acc = K;            yout = y + a*K        (a = 0.5 dt)
 //   s=2: acc = acc + 2.0*K;  yout = y + a*K        (a = 0.5 dt)
 //   s=3: acc = acc + 2.0*K;  yout = y + a*K        (a = dt)
 //   s=4:                     yout = y + a*(K+acc)  (a = dt/6, yout == y)
-//  stage-value form (split-lane FAST variant): no accumulator; the slopes are
+//  stage-value form (FAST kernels): no accumulator; the slopes are
 //  recovered from the stage states, K1 = 2(Y2-y)/dt, K2 = 2(Y3-y)/dt,
 //  K3 = (Y4-y)/dt, so
 //   s=1..3: yout = y + a*K   (Y2, Y3, Y4)
@@ -108,14 +125,12 @@ hipError_t launch_stage_strict(const StageArgs &a, int stage, hipStream_t s);
 hipError_t launch_rhs_strict(const RhsArgs &a, hipStream_t s);
 const char *stage_strict_kernel_name(int stage);
 
-// FAST mode: returns hipErrorInvalidValue for unsupported shapes.  variant 1
-// = one lane per velocity bin (8 fields per lane), variant 2 = two lanes per
-// bin (nu and nubar halves).
-// nteams > 0 (split-lane variant only): nteams teams, each streaming one
+// FAST mode, one RK4 stage per launch (variant 2, stage_fast.hip): two lanes
+// per velocity bin (nu and nubar halves).  Returns hipErrorInvalidValue for
+// unsupported shapes.  nteams > 0: nteams teams, each streaming one
 // contiguous z range; otherwise chunks of z_chunk cells.
-hipError_t launch_stage_fast(const StageArgs &a, int stage, int z_chunk, int variant, int nteams,
-                             hipStream_t s);
-hipError_t launch_rhs_fast(const RhsArgs &a, int z_chunk, int variant, int nteams, hipStream_t s);
+hipError_t launch_stage_fast(const StageArgs &a, int stage, int z_chunk, int nteams, hipStream_t s);
+hipError_t launch_rhs_fast(const RhsArgs &a, int z_chunk, int nteams, hipStream_t s);
 // teams of the split-lane kernel resident on the current device at once
 int fast_resident_teams(int VP, int scheme, int bc);
 
@@ -165,7 +180,7 @@ const char *stage_quad_kernel_name();
 // run of the launch has the matter term.
 hipError_t launch_stage_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
                                    int L, bool uniform, bool mat, hipStream_t s);
-const char *stage_fast_kernel_name(int VP, int stage, int variant);
+const char *stage_fast_kernel_name(int VP, int stage);
 // Batched FAST stage over `nitems` work items (run index, first z cell) of
 // runs with the same VP and scheme; `runs` and `items` are device arrays.
 hipError_t launch_stage_fast_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,

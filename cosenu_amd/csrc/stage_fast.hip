@@ -37,227 +37,6 @@ using namespace fastk;
 
 constexpr int kWin = 8; // u(j-3) .. u(j+4)
 
-template <int W, int SCHEME, int STAGE>
-__global__ __launch_bounds__(256) void cosenu_fast_stage(StageArgs A, int L, int nchunks)
-{
-    constexpr int TEAMS = (W >= 4) ? 1 : 4 / W;
-    __shared__ double ring[TEAMS][8][W][6];
-
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int team = wave / W, wit = wave % W;
-    const int v = wit * 64 + lane;
-    const int VP = A.VP, nz = A.nz;
-    const size_t zs = (size_t)8 * VP;
-
-    // XCD-aware chunk order: blocks are dealt round-robin to the 8 XCDs, so
-    // give each XCD a contiguous run of chunks (neighbours share halo rows in
-    // that XCD's L2).
-    const int nb = gridDim.x;
-    const int b = blockIdx.x;
-    const int per = nb / 8, rem = nb % 8, x = b % 8, k = b / 8;
-    const int bb = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
-    const int chunk = bb * TEAMS + team;
-    const bool active = chunk < nchunks;
-    const int j0 = (active ? chunk : 0) * L;
-
-    const double vi = A.vz[v];
-    const bool pos = vi > 0.0;
-    const bool store = active && v < A.nvz;
-    const double *in = A.yin + v;
-
-    auto zrow = [&](int m) -> int {
-        int w = m % nz;
-        return w < 0 ? w + nz : w;
-    };
-
-    double win[8][kWin];
-#pragma unroll
-    for (int kk = 0; kk < kWin; kk++) {
-        const double *r = in + (size_t)zrow(j0 - 4 + kk) * zs;
-#pragma unroll
-        for (int f = 0; f < 8; f++)
-            win[f][kk] = r[(size_t)f * VP];
-    }
-
-    // padding bins (v >= nvz) may hold stale staging data: keep them out
-    const bool live = v < A.nvz;
-    auto moments_to_ring = [&](int kk, int zabs) {
-        double S = live ? win[EXI][kk] + win[BEXI][kk] : 0.0;
-        double R = live ? win[EXR][kk] - win[BEXR][kk] : 0.0;
-        double D = live ? (win[EE][kk] - win[XX][kk]) - (win[BEE][kk] - win[BXX][kk]) : 0.0;
-        double m0 = wave_sum(S), m1 = wave_sum(vi * S);
-        double m2 = wave_sum(R), m3 = wave_sum(vi * R);
-        double m4 = wave_sum(D), m5 = wave_sum(vi * D);
-        if (lane == 0) {
-            double *slot = ring[team][zabs & 7][wit];
-            slot[0] = m0; slot[1] = m1; slot[2] = m2; slot[3] = m3; slot[4] = m4; slot[5] = m5;
-        }
-    };
-
-    const bool coll = (A.flags & kFlagColl) != 0;
-    if (coll) {
-        moments_to_ring(4, j0);
-        moments_to_ring(5, j0 + 1);
-        moments_to_ring(6, j0 + 2);
-    }
-
-    const double fac = -vi / A.dz;                 // FV advection
-    const double fdfac = -vi / (12.0 * A.dz);      // FD advection
-    const double ko_eps = -A.ko / A.dz / 16.0;     // FD dissipation (3rd order)
-    const double mudv = A.mu * A.dv;
-    const double st = A.st * A.pmo, ct = A.ct * A.pmo;
-    const bool vac = (A.flags & kFlagVac) != 0, mat = (A.flags & kFlagMat) != 0;
-    const bool advect = !(A.flags & kFlagAdvecOff);
-
-    double Fprev[8];
-#pragma unroll
-    for (int f = 0; f < 8; f++)
-        Fprev[f] = 0.0;
-
-    for (int j = j0 - 1; j < j0 + L; j++) {
-        // prefetch the row entering the window next iteration
-        double nxt[8];
-        {
-            const double *r = in + (size_t)zrow(j + 5) * zs;
-#pragma unroll
-            for (int f = 0; f < 8; f++)
-                nxt[f] = r[(size_t)f * VP];
-        }
-        const bool cell = j >= j0;
-        const size_t q = (size_t)zrow(j) * zs + v;
-        double yv[8], av[8];
-        if (cell && STAGE >= 2) {
-#pragma unroll
-            for (int f = 0; f < 8; f++) {
-                yv[f] = A.y[q + (size_t)f * VP];
-                av[f] = A.acc[q + (size_t)f * VP];
-            }
-        }
-
-        if (coll)
-            moments_to_ring(7, j + 4);
-
-        double F[8];
-        if (SCHEME == 0 && advect) {
-#pragma unroll
-            for (int f = 0; f < 8; f++) {
-                const double *w = win[f];
-                double a0 = pos ? w[0] : w[7], a1 = pos ? w[1] : w[6], a2 = pos ? w[2] : w[5];
-                double a3 = pos ? w[3] : w[4], a4 = pos ? w[4] : w[3], a5 = pos ? w[5] : w[2];
-                double a6 = pos ? w[6] : w[1];
-                F[f] = weno_up(a0, a1, a2, a3, a4, a5, a6);
-            }
-        }
-
-        if (W > 1)
-            __syncthreads();
-
-        if (cell) {
-            double K[8];
-            // local values u(j) = window slot 3
-            const double ee = win[EE][3], xx = win[XX][3], exr = win[EXR][3], exi = win[EXI][3];
-            const double bee = win[BEE][3], bxx = win[BXX][3], bexr = win[BEXR][3],
-                         bexi = win[BEXI][3];
-#pragma unroll
-            for (int f = 0; f < 8; f++)
-                K[f] = 0.0;
-            if (advect) {
-                if (SCHEME == 0) {
-#pragma unroll
-                    for (int f = 0; f < 8; f++)
-                        K[f] = fac * (F[f] - Fprev[f]);
-                } else {
-#pragma unroll
-                    for (int f = 0; f < 8; f++) {
-                        const double *w = win[f]; // w[1..5] = u(j-2..j+2)
-                        K[f] = fdfac * ((w[1] - w[5]) - 8.0 * (w[2] - w[4]));
-                    }
-                }
-            }
-            if (vac) {
-                K[EE] += -2.0 * st * exi;
-                K[XX] += 2.0 * st * exi;
-                K[EXR] += -2.0 * ct * exi;
-                K[EXI] += 2.0 * ct * exr + st * (ee - xx);
-                K[BEE] += -2.0 * st * bexi;
-                K[BXX] += 2.0 * st * bexi;
-                K[BEXR] += -2.0 * ct * bexi;
-                K[BEXI] += 2.0 * ct * bexr + st * (bee - bxx);
-            }
-            if (mat) {
-                const double h = A.Hm[zrow(j)];
-                K[EXR] += h * exi;
-                K[EXI] -= h * exr;
-                K[BEXR] -= h * bexi;
-                K[BEXI] += h * bexr;
-            }
-            if (coll) {
-                double M[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-                for (int w = 0; w < W; w++) {
-                    const double *slot = ring[team][j & 7][w];
-#pragma unroll
-                    for (int m = 0; m < 6; m++)
-                        M[m] += slot[m];
-                }
-                const double S = mudv * (M[0] - vi * M[1]);
-                const double R = mudv * (M[2] - vi * M[3]);
-                const double D = mudv * (M[4] - vi * M[5]);
-                const double Iee = 2.0 * (exr * S - exi * R);
-                const double Ibee = 2.0 * (bexr * S + bexi * R);
-                K[EE] += Iee;
-                K[XX] -= Iee;
-                K[EXR] += (xx - ee) * S + exi * D;
-                K[EXI] += (ee - xx) * R - exr * D;
-                K[BEE] += Ibee;
-                K[BXX] -= Ibee;
-                K[BEXR] += (bxx - bee) * S - bexi * D;
-                K[BEXI] += (bxx - bee) * R + bexr * D;
-            }
-            if (SCHEME == 1) {
-                const bool ko5 = (A.flags & kFlagKO5) != 0;
-#pragma unroll
-                for (int f = 0; f < 8; f++) {
-                    const double *w = win[f]; // w[k] = u(j-3+k)
-                    if (ko5)
-                        K[f] += (-A.ko / A.dz / 64.0)
-                              * ((w[0] + w[6]) - 6.0 * (w[1] + w[5]) + 15.0 * (w[2] + w[4]) - 20.0 * w[3]);
-                    else
-                        K[f] += ko_eps * ((w[1] + w[5]) - 4.0 * (w[2] + w[4]) + 6.0 * w[3]);
-                }
-            }
-            if (store) {
-#pragma unroll
-                for (int f = 0; f < 8; f++) {
-                    const size_t o = q + (size_t)f * VP;
-                    if (STAGE == 0) {
-                        A.yout[o] = K[f];
-                    } else if (STAGE == 1) { // y is the stage input: window slot 3
-                        A.acc[o] = K[f];
-                        A.yout[o] = win[f][3] + A.a * K[f];
-                    } else if (STAGE == 4) {
-                        A.yout[o] = yv[f] + A.a * (K[f] + av[f]);
-                    } else {
-                        A.acc[o] = av[f] + 2.0 * K[f];
-                        A.yout[o] = yv[f] + A.a * K[f];
-                    }
-                }
-            }
-        }
-
-        // slide the window
-#pragma unroll
-        for (int f = 0; f < 8; f++) {
-#pragma unroll
-            for (int kk = 0; kk < kWin - 1; kk++)
-                win[f][kk] = win[f][kk + 1];
-            win[f][kWin - 1] = nxt[f];
-            if (SCHEME == 0)
-                Fprev[f] = F[f];
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Split-lane variant.  The 8 fields of a velocity bin are shared by two lanes
 // of one wavefront: lanes 0-31 carry (ee, xx, ex_re, ex_im) of 32 bins and
@@ -626,45 +405,15 @@ int resident_teams_scheme(int VP)
     }
 }
 
-template <int W, int SCHEME>
-hipError_t launch_w(const StageArgs &a, int stage, int L, hipStream_t s)
-{
-    constexpr int TEAMS = (W >= 4) ? 1 : 4 / W;
-    const int nchunks = a.nz / L;
-    dim3 grid((nchunks + TEAMS - 1) / TEAMS), block(64 * W * TEAMS);
-    switch (stage) {
-    case 0: hipLaunchKernelGGL((cosenu_fast_stage<W, SCHEME, 0>), grid, block, 0, s, a, L, nchunks); break;
-    case 1: hipLaunchKernelGGL((cosenu_fast_stage<W, SCHEME, 1>), grid, block, 0, s, a, L, nchunks); break;
-    case 2: hipLaunchKernelGGL((cosenu_fast_stage<W, SCHEME, 2>), grid, block, 0, s, a, L, nchunks); break;
-    case 3: // stages 2 and 3 run the same code with their own coefficient
-        hipLaunchKernelGGL((cosenu_fast_stage<W, SCHEME, 2>), grid, block, 0, s, a, L, nchunks); break;
-    case 4: hipLaunchKernelGGL((cosenu_fast_stage<W, SCHEME, 4>), grid, block, 0, s, a, L, nchunks); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-template <int SCHEME>
-hipError_t launch_scheme(const StageArgs &a, int stage, int L, hipStream_t s)
-{
-    switch (a.VP) {
-    case 64: return launch_w<1, SCHEME>(a, stage, L, s);
-    case 128: return launch_w<2, SCHEME>(a, stage, L, s);
-    case 256: return launch_w<4, SCHEME>(a, stage, L, s);
-    default: return hipErrorInvalidValue;
-    }
-}
-
 } // namespace
 
 int fast_supported(int VP, int bc) { return (VP == 64 || VP == 128 || VP == 256) && (bc == 0 || bc == 1); }
 
-hipError_t launch_stage_fast(const StageArgs &a, int stage, int L, int variant, int nteams,
-                             hipStream_t s)
+hipError_t launch_stage_fast(const StageArgs &a, int stage, int L, int nteams, hipStream_t s)
 {
     if (!fast_supported(a.VP, a.bc))
         return hipErrorInvalidValue;
-    if (nteams > 0 && variant != 1) {
+    if (nteams > 0) {
         if (nteams > a.nz)
             return hipErrorInvalidValue;
     } else if (L < 1 || a.nz % L != 0) {
@@ -673,8 +422,6 @@ hipError_t launch_stage_fast(const StageArgs &a, int stage, int L, int variant, 
     if (a.bc == 1) // open boundary: split-lane kernel only
         return a.scheme == 0 ? launch_scheme2<0, true>(a, stage, L, nteams, s)
                              : launch_scheme2<1, true>(a, stage, L, nteams, s);
-    if (variant == 1)
-        return a.scheme == 0 ? launch_scheme<0>(a, stage, L, s) : launch_scheme<1>(a, stage, L, s);
     return a.scheme == 0 ? launch_scheme2<0, false>(a, stage, L, nteams, s)
                          : launch_scheme2<1, false>(a, stage, L, nteams, s);
 }
@@ -686,7 +433,7 @@ int fast_resident_teams(int VP, int scheme, int bc)
     return scheme == 0 ? resident_teams_scheme<0, false>(VP) : resident_teams_scheme<1, false>(VP);
 }
 
-hipError_t launch_rhs_fast(const RhsArgs &r, int L, int variant, int nteams, hipStream_t s)
+hipError_t launch_rhs_fast(const RhsArgs &r, int L, int nteams, hipStream_t s)
 {
     StageArgs a;
     a.yin = r.in; a.y = nullptr; a.acc = nullptr; a.yout = r.out;
@@ -695,7 +442,7 @@ hipError_t launch_rhs_fast(const RhsArgs &r, int L, int variant, int nteams, hip
     a.nz = r.nz; a.nvz = r.nvz; a.VP = r.VP; a.flags = r.flags; a.bc = r.bc; a.scheme = r.scheme;
     a.a = 0.0;
     a.dz = r.dz; a.dv = r.dv; a.ct = r.ct; a.st = r.st; a.pmo = r.pmo; a.mu = r.mu; a.ko = r.ko;
-    return launch_stage_fast(a, 0, L, variant, nteams, s);
+    return launch_stage_fast(a, 0, L, nteams, s);
 }
 
 namespace {
@@ -739,14 +486,11 @@ hipError_t launch_stage_fast_batch(const BatchRun *runs, const int2 *items, int 
                        : launch_batch_scheme<1>(runs, items, nitems, VP, L, stage, s);
 }
 
-const char *stage_fast_kernel_name(int VP, int stage, int variant)
+const char *stage_fast_kernel_name(int VP, int stage)
 {
     static char buf[96];
     const int st = stage + 1 == 3 ? 2 : stage + 1;
-    if (variant == 1)
-        snprintf(buf, sizeof(buf), "cosenu_fast_stage<%d, *, %d>", VP / 64, st);
-    else
-        snprintf(buf, sizeof(buf), "cosenu_fast2_stage<%d, *, %d>", VP / 32, st);
+    snprintf(buf, sizeof(buf), "cosenu_fast2_stage<%d, *, %d>", VP / 32, st);
     return buf;
 }
 

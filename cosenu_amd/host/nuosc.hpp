@@ -6,7 +6,7 @@
 //   state.bin   int32 iteration, then for i < nvz, j < nz: 8 doubles
 //               (ee, xx, ex_re, ex_im, bee, bxx, bex_re, bex_im)   snaps.hpp:6-39
 //   state0.bin  the same body without the header                   snaps.hpp:43-71
-//   G0.bin      for i < nvz, j < nz: (G, bG)                        initialize.hpp:231-232
+//   G0.bin      for i < nvz, j < nz: (G, bG)                        initialize.hpp:40-41
 //   rho_<t>.dat text, j outer, i inner, 16 significant digits       snaps.hpp:187-218
 //   conserved_quantities.dat, survival_probability.dat              analysis.hpp:71-185
 #pragma once

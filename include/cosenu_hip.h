@@ -86,8 +86,8 @@ typedef struct cosenu_info {
                             periodic, nvz <= 256), 4 = two RK4 stages per
                             launch (default otherwise), 3 = barrier-free
                             (default for small problems), 2 = one stage per
-                            launch, two lanes per bin (nu / nubar halves), 1 =
-                            one lane per bin; 0 in STRICT.  nvz > 256: 3 only
+                            launch, two lanes per bin (nu / nubar halves;
+                            the open boundary); 0 in STRICT.  nvz > 256: 3 only
                             (periodic, nvz a multiple of 64 with every group
                             of 32 bins on one side of v = 0; otherwise
                             cosenu_create returns COSENU_ENOTSUP and such grids
@@ -126,14 +126,14 @@ int cosenu_destroy(cosenu_ctx *ctx);
 int cosenu_get_info(cosenu_ctx *ctx, cosenu_info *info);
 
 /* The angular profiles G0->G, G0->bG (structures.hpp:124-137), set by
- * NuOsc::initialize (initialize.hpp:218-219) or NuOsc::read_G0
+ * NuOsc::initialize (initialize.hpp:27-28) or NuOsc::read_G0
  * (snaps.hpp:161-183).  Planes of nvz rows as described above. */
 int cosenu_set_profile(cosenu_ctx *ctx, const double *G, const double *bG, ptrdiff_t row_stride,
                        int row_offset);
 
 /* State transfers.  `which` = COSENU_STATE or COSENU_STATE0.  Uploading
  * COSENU_STATE replaces the v_stat that initialize/read_state fill
- * (initialize.hpp:221-229, snaps.hpp:75-124); COSENU_STATE0 replaces the
+ * (initialize.hpp:30-38, snaps.hpp:75-124); COSENU_STATE0 replaces the
  * v_stat0 copy (main.cpp:131-150). */
 int cosenu_upload_planes(cosenu_ctx *ctx, int which, const double *const planes[8],
                          ptrdiff_t row_stride, int row_offset);
@@ -143,7 +143,7 @@ int cosenu_upload_interleaved(cosenu_ctx *ctx, int which, const double *buf);
 int cosenu_download_interleaved(cosenu_ctx *ctx, int which, double *buf);
 
 /* Initial states built on the device (the reference builds them on the host,
- * initialize.hpp:214-234).  Every initial state of the reference and of the
+ * initialize.hpp:23-43).  Every initial state of the reference and of the
  * benchmark workloads is separable: field f at (z, v) is a[f*nvz + v] *
  * b[f*nz + z], one rounded product, exactly what the host computes (e.g.
  * ee = (0.5 G(v)) (1 + sqrt(1 - eps(z)^2))).  a is 8 x nvz, b is 8 x nz.
@@ -155,8 +155,15 @@ int cosenu_set_profile_v(cosenu_ctx *ctx, const double *G, const double *bG);
  * copy_state(v_stat, v_stat0) after initialize (main.cpp:139). */
 int cosenu_copy_state(cosenu_ctx *ctx, int from, int to);
 
-/* NuOsc::step_rk4 (nuosc.hpp:320-348) nsteps times.  Enqueues only. */
+/* NuOsc::step_rk4 (nuosc.hpp:320-348) nsteps times.  Enqueues only.  Runs of
+ * at least COSENU_GRAPH_STEPS steps (environment, default 16) replay a
+ * captured HIP graph of that many steps; the rest are launched one by one. */
 int cosenu_step(cosenu_ctx *ctx, int nsteps);
+/* Capture now the graph that cosenu_step replays (no counterpart in the
+ * reference): the first long cosenu_step then pays no capture and
+ * instantiation.  Runs nothing on the device; a no-op while per-launch timing
+ * is enabled or graphs are switched off (COSENU_GRAPH_STEPS=0). */
+int cosenu_prepare(cosenu_ctx *ctx);
 /* Block until all work of the context has finished. */
 int cosenu_sync(cosenu_ctx *ctx);
 
@@ -201,7 +208,7 @@ const char *cosenu_stage_kernel_name(cosenu_ctx *ctx, int stage);
  * Autotuner (no counterpart in the reference).  Times whole FAST RK4 steps
  * for each kernel variant and z-chunk length that divides nz on a
  * scratch context of p's shape, and records the fastest; contexts created
- * afterwards with the same (nvz rounded to 64, scheme, nz) use it.  When the
+ * afterwards with the same (padded bins, scheme, nz) use it.  When the
  * environment variable COSENU_TUNE_FILE names a file, the table is read from
  * and written to it (text: VP scheme nz variant z_chunk ms). */
 int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *ms);
@@ -209,7 +216,7 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
 /* ------------------------------------------------------------------------
  * Batched runs (no counterpart in the reference, whose sweeps are one process
  * per job, manage.py:361-428).  Many independent FAST-mode runs, of uneven nz,
- * nvz and physics, advance together: runs are grouped by (nvz rounded to 64,
+ * nvz and physics, advance together: runs are grouped by (padded bins,
  * scheme), each group's states are packed into shared buffers, and one launch
  * per RK stage and group covers all (run, z-chunk) work items; runs of
  * cosenu_batch_step are replayed from captured HIP graphs.  Every run is
@@ -236,7 +243,7 @@ int cosenu_batch_get_info(cosenu_batch *b, cosenu_batch_info *info);
  * sweeps are one process per job, manage.py:361-428).  Builds a scratch
  * batch of the given runs, times whole FAST steps for work-item chunks of
  * 32..512 z cells (at most a group's largest nz; a run's last item is
- * shorter) and records the fastest for every (nvz rounded to 64, scheme)
+ * shorter) and records the fastest for every (padded bins, scheme)
  * group of the runs in the tune table (entry nz = -1; COSENU_TUNE_FILE as for
  * cosenu_autotune).  Batches created afterwards use it.  Needs the device
  * memory of the batch; *ms is per batch step. */

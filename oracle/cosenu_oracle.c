@@ -253,24 +253,33 @@ static inline void collective_brute(const grid_t *g, const fields_t *in, int i, 
 }
 
 /* Moment form: sum_k (1 - v_i v_k) f_k = F0 - v_i F1 for f in {S, R, D}.
- * mom[j*6 + {0..5}] = F0S, F1S, F0R, F1R, F0D, F1D. */
+ * mom[j*6 + {0..5}] = F0S, F1S, F0R, F1R, F0D, F1D.  Each cell's sums run
+ * over k = 0..nvz-1 in order; blocks of 64 z cells walk the rows (k outer)
+ * so that the z-contiguous rows are read sequentially. */
 static void moments(const grid_t *g, const fields_t *in, double *mom)
 {
-#pragma omp parallel for
-    for (int j = 0; j < g->nz; j++) {
-        double m[6] = {0, 0, 0, 0, 0, 0};
+    const int nb = (g->nz + 63) / 64;
+#pragma omp parallel for schedule(static)
+    for (int blk = 0; blk < nb; blk++) {
+        const int j0 = blk * 64, j1 = j0 + 64 < g->nz ? j0 + 64 : g->nz;
+        double m[64][6];
+        memset(m, 0, sizeof(m));
         for (int k = 0; k < g->nvz; k++) {
-            size_t q = AT(g, k, j);
-            double S = in->f[3][q] + in->f[7][q];
-            double Rr = in->f[2][q] - in->f[6][q];
-            double D = in->f[0][q] - in->f[1][q] - in->f[4][q] + in->f[5][q];
-            double v = g->vz[k];
-            m[0] += S; m[1] += v * S;
-            m[2] += Rr; m[3] += v * Rr;
-            m[4] += D; m[5] += v * D;
+            const double v = g->vz[k];
+            for (int j = j0; j < j1; j++) {
+                size_t q = AT(g, k, j);
+                double S = in->f[3][q] + in->f[7][q];
+                double Rr = in->f[2][q] - in->f[6][q];
+                double D = in->f[0][q] - in->f[1][q] - in->f[4][q] + in->f[5][q];
+                double *mm = m[j - j0];
+                mm[0] += S; mm[1] += v * S;
+                mm[2] += Rr; mm[3] += v * Rr;
+                mm[4] += D; mm[5] += v * D;
+            }
         }
-        for (int q = 0; q < 6; q++)
-            mom[(size_t)j * 6 + q] = m[q];
+        for (int j = j0; j < j1; j++)
+            for (int q = 0; q < 6; q++)
+                mom[(size_t)j * 6 + q] = m[j - j0][q];
     }
 }
 

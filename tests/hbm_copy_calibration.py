@@ -1,6 +1,6 @@
 """Plain device copy of a known size, for calibrating the HBM byte counters.
 
-    rocprofv3 --pmc FETCH_SIZE -- python3 -m cosenu_amd.copy_probe
+    rocprofv3 --pmc FETCH_SIZE -- python3 tests/hbm_copy_calibration.py
 
 The copy kernel reads and writes exactly 512 MiB per launch; the ratio of
 the counter's reading to that figure is the scale applied to the same

@@ -70,3 +70,22 @@ def test_invalid_parameters_are_returned_before_any_device_call(kw, match):
     import cosenu_amd as C
     with pytest.raises(C.CosenuError, match=match):
         C.Simulation(C.RunParams(mode=C.FAST, **kw))
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+@pytest.mark.parametrize("nvz", [150, 192, 255])
+@pytest.mark.parametrize("bc", [0, 1])
+def test_fast_mode_takes_129_to_256_bins_with_either_boundary(nvz, bc):
+    """Rows of 129..256 bins are padded to 256, a width every FAST kernel
+    takes: validation accepts them with either boundary.  Without a GPU the
+    call then fails at the first device call, not at validation."""
+    import cosenu_amd as C
+    try:
+        sim = C.Simulation(C.RunParams(nz=64, nvz=nvz, bc=bc, mode=C.FAST))
+    except C.CosenuError as e:
+        assert "hipSetDevice" in str(e) or "hip" in str(e).lower(), str(e)
+        assert "FAST mode" not in str(e)
+    else:
+        assert sim.info()["fast_variant"] in (2, 4)
+        sim.close()
+

@@ -1,0 +1,155 @@
+"""STRICT mode at the velocity widths where it stands in for the reference.
+
+STRICT mode is bit-identical to the oracle, and the oracle to the reference
+build (tests/test_oracle_ref.py).  The STRICT stage kernel stages ZB =
+max(1, 256/VP) z rows of a block in LDS and its thread mapping changes with
+the row width VP, so the bit-identity is pinned here at every width the long
+runs use it for (nvz = 128 for nz = 65536, 256 for the nz = 2^21 physics, 512
+and 1024 for the wide grids): one RHS and ten RK4 steps, FV and FD, periodic
+and open boundaries, with and without the matter term (rhs_fv.hpp:115-236,
+rhs_fd.hpp:6-148, nuosc.hpp:212-348).
+
+FAST mode on grids that put a bin at v = 0 (odd nvz: the reference's
+advection factor -v/dz is then zero for that bin, rhs_fv.hpp:149-161) and on
+grids of 129..256 bins (rows padded to 256), and the state copy of
+NuOsc::copy_state (nuosc.hpp:193-210), complete the set.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import cosenu_amd as C
+from cosenu_amd.initial import seeded_ic
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(scheme, bc, flags, nz, nvz, mode, omega=0.3, theta=37.0):
+    kw = dict(nz=nz, nvz=nvz, z0=-nz / 2.0, z1=nz / 2.0, scheme=scheme, bc=bc, flags=flags,
+              omega=omega, theta=theta, pmo=1.0, mu=1.0)
+    return O.RunParams(**kw), C.RunParams(mode=mode, **kw)
+
+
+def _state(po, seed, amp=1e-3, noise=1e-4):
+    G, bG, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=seed, amp=amp)
+    rng = np.random.default_rng(seed)
+    # a z-dependent background so that the boundary rules and the stencils matter
+    s = s * (1.0 + 0.05 * np.sin(np.linspace(0, 3.0, po.nz)))[None, None, :]
+    return G, bG, s + noise * rng.standard_normal(s.shape)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+WIDE_CASES = [
+    (O.FV, O.PERIODIC, O.VAC | O.COLL),
+    (O.FD, O.PERIODIC, O.VAC | O.COLL),
+    (O.FV, O.OPEN, O.VAC | O.COLL),
+    (O.FD, O.PERIODIC, O.VAC | O.MAT | O.COLL | O.KO5),
+    (O.FV, O.PERIODIC, O.VAC | O.MAT | O.COLL),
+]
+
+
+@pytest.mark.parametrize("nvz", [128, 256, 512, 1024])
+@pytest.mark.parametrize("scheme,bc,flags", WIDE_CASES)
+def test_strict_wide_rhs_bitwise(gpu, scheme, bc, flags, nvz):
+    po, pc = _pair(scheme, bc, flags, 32, nvz, C.STRICT)
+    _, _, s = _state(po, 51)
+    ref = O.rhs(po, s)
+    with C.Simulation(pc) as sim:
+        sim.upload(s)
+        out = sim.rhs()
+    print(f"STRICT rhs nvz={nvz} scheme={scheme} bc={bc} flags={flags}: "
+          f"equal={np.array_equal(out, ref)}", flush=True)
+    assert np.array_equal(out, ref), _rel(out, ref)
+
+
+@pytest.mark.parametrize("nvz", [128, 256, 512, 1024])
+@pytest.mark.parametrize("scheme,bc,flags", WIDE_CASES)
+def test_strict_wide_rk4_bitwise(gpu, scheme, bc, flags, nvz):
+    po, pc = _pair(scheme, bc, flags, 32, nvz, C.STRICT)
+    _, _, s = _state(po, 52)
+    ref = O.step(po, s, 10)
+    with C.Simulation(pc) as sim:
+        sim.upload(s)
+        sim.step(10)
+        out = sim.download()
+    print(f"STRICT rk4 nvz={nvz} scheme={scheme} bc={bc} flags={flags}: "
+          f"equal={np.array_equal(out, ref)}", flush=True)
+    assert np.array_equal(out, ref), _rel(out, ref)
+
+
+@pytest.mark.parametrize("nvz", [33, 65])
+@pytest.mark.parametrize("scheme,bc,flags", [(O.FV, O.PERIODIC, O.VAC | O.COLL),
+                                             (O.FD, O.PERIODIC, O.VAC | O.COLL),
+                                             (O.FV, O.OPEN, O.VAC | O.COLL),
+                                             (O.FV, O.PERIODIC, O.VAC | O.MAT | O.COLL)])
+def test_fast_odd_nvz_bin_at_v_zero(gpu, scheme, bc, flags, nvz):
+    """Odd nvz on [-1, 1] puts bin (nvz-1)/2 at v = 0 exactly; its advection
+    term is zero in the reference (fac = -v/dz = 0) and must stay so in the
+    upwind kernels, which pick one stencil order per group of 32 bins."""
+    po, pc = _pair(scheme, bc, flags, 96, nvz, C.FAST)
+    _, _, smooth = seeded_ic(po, 0.6, 0.5, 0.9, seed=53, amp=1e-3)
+    vz = pc.vz()
+    assert vz[(nvz - 1) // 2] == 0.0
+    with C.Simulation(pc) as sim:
+        sim.upload(smooth)
+        r = sim.rhs()
+        assert _rel(r, O.rhs(po, smooth)) <= 1e-12
+        sim.upload(smooth)
+        sim.step(30)
+        out = sim.download()
+        variant = sim.info()["fast_variant"]
+    ref = O.step(po, smooth, 30)
+    print(f"FAST odd nvz={nvz} scheme={scheme} bc={bc}: variant {variant}, "
+          f"rel {_rel(out, ref):.3e}", flush=True)
+    assert _rel(out, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("nvz", [150, 192, 255])
+@pytest.mark.parametrize("scheme,bc", [(O.FV, O.PERIODIC), (O.FD, O.PERIODIC), (O.FV, O.OPEN)])
+def test_fast_padded_rows_129_to_256_bins(gpu, scheme, bc, nvz):
+    """129..256 bins take rows of 256 with zero padding bins, on every FAST
+    kernel (the two- and four-stage kernels included), either boundary."""
+    po, pc = _pair(scheme, bc, O.VAC | O.COLL, 64, nvz, C.FAST)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=54, amp=1e-3)
+    ref = O.step(po, s, 20)
+    with C.Simulation(pc) as sim:
+        info = sim.info()
+        sim.upload(s)
+        sim.step(20)
+        out = sim.download()
+    expect = 2 if bc == O.OPEN else (5 if scheme == O.FD else 4)
+    print(f"FAST nvz={nvz} scheme={scheme} bc={bc}: variant {info['fast_variant']}, "
+          f"rel {_rel(out, ref):.3e}", flush=True)
+    assert info["fast_variant"] == expect
+    assert _rel(out, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("mode", [C.STRICT, C.FAST])
+def test_copy_state(gpu, mode):
+    """cosenu_copy_state copies the interior of one device state to the other
+    (NuOsc::copy_state, nuosc.hpp:193-210, as main.cpp:139 uses it for
+    v_stat0), in both directions, and a copy onto itself changes nothing."""
+    po, pc = _pair(O.FV, O.PERIODIC, O.VAC | O.COLL, 64, 40, mode)
+    G, bG, s = _state(po, 55)
+    with C.Simulation(pc) as sim:
+        sim.set_profile(G, bG)
+        sim.upload(s)
+        sim.copy_state(C.STATE, C.STATE0)
+        assert np.array_equal(sim.download(C.STATE0), s)
+        d = sim.diagnostics()
+        assert d["dP_max"] == 0.0 and d["Pee"] == 1.0
+        sim.step(5)
+        s5 = sim.download()
+        assert not np.array_equal(s5, s)
+        sim.copy_state(C.STATE, C.STATE)
+        assert np.array_equal(sim.download(), s5)
+        sim.copy_state(C.STATE0, C.STATE)  # back to t = 0
+        assert np.array_equal(sim.download(), s)
+        assert np.array_equal(sim.download(C.STATE0), s)
+        sim.step(5)  # the stepping continues from the copied state
+        assert np.array_equal(sim.download(), s5)
+        with pytest.raises(C.CosenuError, match="copy_state"):
+            sim.copy_state(C.STATE, 7)

@@ -15,10 +15,11 @@ over many ELN configurations can skip most solver runs.
   with FP32 accumulation on the matrix cores; forward, backward and AdamW are
   the hand-written kernels of csrc/mlp.hip (cosenu_mlp_*).
 * Fallback: an ensemble of independently initialised networks; the mean
-  over bins of the ensemble standard deviation is the per-sample error
-  indicator.  Its acceptance threshold is calibrated on held-out solver runs
-  so that accepted samples keep the mean |dP_ee| within the bound (2%);
-  every rejected sample is run by the solver.
+  over bins of the ensemble standard deviation is the per-run error
+  indicator.  Its acceptance threshold is calibrated on solver runs held out
+  of training so that no calibration run whose own error (mean over bins of
+  |dP_ee|) exceeds the bound (2%) is accepted, and measured on a second
+  held-out set; every rejected run is run by the solver.
 """
 from __future__ import annotations
 
@@ -263,33 +264,85 @@ def train_member(X, Y, cfg: TrainConfig, seed: int, net: MLP | None = None, log=
     return net, losses
 
 
-class Surrogate:
-    """Ensemble of trained networks with a calibrated acceptance threshold."""
+def run_errors(pred, Y):
+    """Per-run error: the mean over the velocity bins of |dP_ee|."""
+    return np.abs(np.asarray(pred, dtype=np.float64) - Y).mean(axis=1)
 
-    def __init__(self, members: list, bound: float = 0.02):
+
+class Surrogate:
+    """Ensemble of trained networks with a per-run acceptance gate.
+
+    The gate is the ensemble spread (mean over bins of the members' standard
+    deviation).  `calibrate` sets its threshold tau on solver runs the
+    networks were not trained on, so that no calibration run whose error
+    exceeds the bound is accepted; `evaluate` then measures the gate on
+    further unseen runs (accept rate, largest accepted error, recall of the
+    runs above the bound).  Every rejected run goes to the solver."""
+
+    def __init__(self, members: list, bound: float = 0.02, meta: dict | None = None):
         self.members = members
         self.bound = bound
         self.tau = -1.0  # acceptance threshold on the ensemble spread
+        self.meta = dict(meta or {})
 
     def predict(self, X):
         P = np.stack([m.forward(X) for m in self.members])
         return P.mean(axis=0), P.std(axis=0).mean(axis=1)
 
     def calibrate(self, X, Y):
-        """Largest spread threshold whose accepted held-out samples keep the
-        mean |dP_ee| within the bound."""
+        """tau just below the smallest spread of a calibration run whose error
+        exceeds the bound (every run's spread if none does)."""
         mean, spread = self.predict(X)
-        err = np.abs(mean - Y).mean(axis=1)
-        order = np.argsort(spread)
-        cum = np.cumsum(err[order]) / np.arange(1, len(order) + 1)
-        ok = np.nonzero(cum <= self.bound)[0]
-        self.tau = float(spread[order][ok[-1]]) if ok.size else -1.0
+        err = run_errors(mean, Y)
+        bad = err > self.bound
+        if bad.any():
+            self.tau = float(np.nextafter(spread[bad].min(), -np.inf))
+        else:
+            self.tau = float(spread.max())
         acc = spread <= self.tau
-        return {"heldout_mean_abs_err_all": float(err.mean()),
-                "heldout_accept_rate": float(acc.mean()),
-                "heldout_mean_abs_err_accepted": float(err[acc].mean()) if acc.any() else None,
-                "tau": self.tau}
+        return {"runs": int(len(err)), "above_bound": int(bad.sum()),
+                "accept_rate": float(acc.mean()),
+                "max_accepted_run_err": float(err[acc].max()) if acc.any() else None,
+                "mean_err_all": float(err.mean()), "tau": self.tau}
+
+    def evaluate(self, X, Y):
+        """The gate on unseen runs: rejected runs take the solver's values."""
+        mean, spread = self.predict(X)
+        err = run_errors(mean, Y)
+        acc = spread <= self.tau
+        bad = err > self.bound
+        screened = np.where(acc, err, 0.0)
+        return {"runs": int(len(err)), "accept_rate": float(acc.mean()),
+                "max_accepted_run_err": float(err[acc].max()) if acc.any() else None,
+                "mean_accepted_run_err": float(err[acc].mean()) if acc.any() else None,
+                "above_bound": int(bad.sum()),
+                "recall_of_above_bound": float((bad & ~acc).sum() / bad.sum()) if bad.any() else None,
+                "accepted_above_bound": int((bad & acc).sum()),
+                "mean_err_network_only": float(err.mean()),
+                "mean_err_screened": float(screened.mean())}
 
     def screen(self, X):
         mean, spread = self.predict(X)
         return mean, spread <= self.tau
+
+    def save(self, path: str):
+        """Ensemble, threshold and metadata in one .npz (no pickles)."""
+        import json
+        params = np.stack([m.get_params() for m in self.members])
+        np.savez_compressed(path, params=params, dims=np.array(self.members[0].dims),
+                            tau=np.float64(self.tau), bound=np.float64(self.bound),
+                            meta=json.dumps(self.meta))
+
+    @classmethod
+    def load(cls, path: str, max_batch: int = 4096, device: int = 0) -> "Surrogate":
+        import json
+        with np.load(path, allow_pickle=False) as z:
+            dims = tuple(int(d) for d in z["dims"])
+            members = []
+            for flat in z["params"]:
+                m = MLP(dims, max_batch=max_batch, device=device)
+                m.set_params(flat)
+                members.append(m)
+            s = cls(members, bound=float(z["bound"]), meta=json.loads(str(z["meta"])))
+            s.tau = float(z["tau"])
+        return s

@@ -1,34 +1,42 @@
 """End-to-end surrogate pipeline on one GPU.
 
-    python -m cosenu_amd.train_surrogate [--runs 2000] [--sweep 256] [--out DIR]
+    python -m cosenu_amd.train_surrogate --nz 8192 --steps 8000 --runs 10000 \
+        --dataset cosenu_amd/data/eln_dataset_nz8192.npz --out DIR
 
 1. Dataset: `--runs` ELN configurations (sigma_nu, sigma_nubar ~ U(0.4, 0.8),
-   alpha ~ U(0.7, 1.1)) advanced by the batched FP64 solver; targets are the
-   time-averaged P_ee(v) over the final 20% of the steps.  90/10 split.
+   alpha ~ U(0.7, 1.1), per-run phase seeds) advanced by the batched FP64
+   solver; targets are P_ee(v) averaged over z and over the final 20% of the
+   steps.  Split 90/10: the first 90% train the networks; the last 10% are
+   held out, half to calibrate the gate and half to measure it.
 2. Training: an ensemble of networks (67 -> 256 -> 256 -> 256 -> 64) trained
    with the library's BF16-MFMA forward/backward/AdamW kernels.
-3. Calibration: the acceptance threshold on the ensemble spread that keeps the
-   held-out mean |dP_ee| within 2%.
+3. Gate: the threshold on the ensemble spread below which no calibration run
+   exceeds the per-run bound (mean over bins of |dP_ee| <= 2%); measured on
+   the test half (accept rate, largest accepted error, recall of the runs
+   above the bound).
 4. Screened sweep: `--sweep` fresh ELN configurations.  Solver-only: the
    batched solver runs all of them.  Screened: the ensemble predicts all of
-   them and the solver runs only the rejected ones.  Reports the speed-up and
-   the mean |dP_ee| of the screened result against the solver's.
+   them and the solver runs only the rejected ones.  Reports the speed-up,
+   the mean |dP_ee| of the screened result against the solver's and the
+   largest error of an accepted run.
 5. Ablation: the trained ensemble against the method without a network
    (predicting the training-set mean P_ee(v)).
 
-Prints one JSON line and writes it (plus the ensemble weights) to --out.
+Prints one JSON line and writes it to DIR/surrogate_report.json, the ensemble
+(weights, threshold) to DIR/surrogate_ensemble.npz.
 
-The dataset can be produced in pieces (one GPU job each):
+The dataset is produced in pieces (one GPU job each), one file per solver
+batch:
 
-    python -m cosenu_amd.train_surrogate --nz 8192 --steps 8000 --runs 2048 \
-        --data-dir D --gen 0 1024
-    ... --gen 1024 2048
-    python -m cosenu_amd.train_surrogate --nz 8192 --steps 8000 --runs 2048 --data-dir D
+    python -m cosenu_amd.train_surrogate --nz 8192 --steps 8000 --runs 10000 \
+        --data-dir D --gen 0 1536
+    ... --gen 1536 3072
 
 Run i of the sample is the same whatever the sample size, so a piece
-`data_<a>_<b>.npz` holds runs [a, b) of every sample with the same seed; the
-full job loads the pieces it finds (same grid, schedule and seed) and solves
-what is missing.
+`data_<a>_<b>.npz` holds runs [a, b) of every sample with the same seed.
+`--collect FILE` gathers the pieces of `--data-dir` into one dataset file
+(ELN parameters and targets); `--dataset FILE` reads such a file, and runs it
+lacks are loaded from `--data-dir` pieces or solved.
 """
 from __future__ import annotations
 
@@ -55,35 +63,62 @@ def _meta(args, setup):
 
 def save_piece(args, setup, a, b):
     """Solve runs [a, b) of the sample and write them to --data-dir."""
-    elns = S.sample_elns(b, args.seed)[a:b]
-    X = np.stack([S.features(e) for e in elns])
-    Y, t = S.solve(elns, setup, log=log)
     os.makedirs(args.data_dir, exist_ok=True)
-    path = os.path.join(args.data_dir, f"data_{a}_{b}.npz")
-    np.savez_compressed(path, X=X, Y=Y, a=a, b=b, solver_s=t,
-                        meta=json.dumps(_meta(args, setup)))
-    log(json.dumps({"piece": [a, b], "solver_s": t, "file": path}))
+    sample = S.sample_elns(b, args.seed)
+    # one file per solver batch, written as soon as the batch is done
+    for a0 in range(a, b, setup.batch):
+        a1 = min(b, a0 + setup.batch)
+        elns = sample[a0:a1]
+        X = np.stack([S.features(e) for e in elns])
+        Y, t = S.solve(elns, setup, log=log)
+        path = os.path.join(args.data_dir, f"data_{a0}_{a1}.npz")
+        np.savez_compressed(path, X=X, Y=Y, a=a0, b=a1, solver_s=t,
+                            meta=json.dumps(_meta(args, setup)))
+        log(json.dumps({"piece": [a0, a1], "solver_s": t, "file": path}))
+
+
+def load_dataset_file(path, args, setup, elns):
+    """Targets from a collected dataset file; checks grid, schedule, seed and
+    the ELN parameters of every run it holds.  Returns (Y, have, solver s)."""
+    n = len(elns)
+    Y = np.full((n, S.NV), np.nan, dtype=np.float32)
+    have = np.zeros(n, dtype=bool)
+    with np.load(path, allow_pickle=False) as z:
+        if json.loads(str(z["meta"])) != _meta(args, setup):
+            raise SystemExit(f"{path}: grid, schedule or seed differ")
+        m = min(n, len(z["Y"]))
+        eln = np.array([[e.sig_nu, e.sig_nubar, e.alpha, e.seed] for e in elns[:m]])
+        if not np.array_equal(eln, z["eln"][:m]):
+            raise SystemExit(f"{path}: ELN parameters differ from the sample")
+        Y[:m] = z["Y"][:m]
+        have[:m] = True
+        t = float(z["solver_s"]) * m / len(z["Y"])
+    return Y, have, t
 
 
 def dataset(args, setup, elns):
-    """Targets of runs [0, len(elns)): pieces found in --data-dir with the same
-    grid, schedule and seed are loaded, the rest solved.
-    Returns (Y, solver seconds, [[a, b, source], ...])."""
+    """Targets of runs [0, len(elns)): a collected dataset file (--dataset),
+    then pieces found in --data-dir with the same grid, schedule and seed,
+    the rest solved.  Returns (Y, solver seconds, [[a, b, source], ...])."""
     n = len(elns)
     Y = np.full((n, S.NV), np.nan, dtype=np.float32)
     have = np.zeros(n, dtype=bool)
     t = 0.0
     pieces = []
     meta = _meta(args, setup)
+    if args.dataset and os.path.exists(args.dataset):
+        Y, have, t = load_dataset_file(args.dataset, args, setup, elns)
+        if have.any():
+            pieces.append([0, int(have.sum()), os.path.basename(args.dataset)])
     if args.data_dir and os.path.isdir(args.data_dir):
         for name in sorted(os.listdir(args.data_dir)):
             if not (name.startswith("data_") and name.endswith(".npz")):
                 continue
-            with np.load(os.path.join(args.data_dir, name)) as z:
+            with np.load(os.path.join(args.data_dir, name), allow_pickle=False) as z:
                 if json.loads(str(z["meta"])) != meta:
                     continue
                 a, b = int(z["a"]), int(z["b"])
-                if a >= n:
+                if a >= n or have[a:min(b, n)].all():
                     continue
                 m = min(b, n) - a
                 X = np.stack([S.features(e) for e in elns[a:a + m]])
@@ -104,21 +139,54 @@ def dataset(args, setup, elns):
     return Y, t, pieces
 
 
+def collect(args, setup, path):
+    """Gather the --data-dir pieces of runs [0, --runs) into one file."""
+    elns = S.sample_elns(args.runs, args.seed)
+    args_dataset, args.dataset = args.dataset, ""
+    n = len(elns)
+    Y = np.full((n, S.NV), np.nan, dtype=np.float32)
+    t = 0.0
+    found = np.zeros(n, dtype=bool)
+    for name in sorted(os.listdir(args.data_dir)):
+        if not (name.startswith("data_") and name.endswith(".npz")):
+            continue
+        with np.load(os.path.join(args.data_dir, name), allow_pickle=False) as z:
+            if json.loads(str(z["meta"])) != _meta(args, setup):
+                continue
+            a, b = int(z["a"]), min(int(z["b"]), n)
+            if a >= n:
+                continue
+            X = np.stack([S.features(e) for e in elns[a:b]])
+            if not np.array_equal(X, z["X"][:b - a]):
+                raise SystemExit(f"{name}: features differ from the sample")
+            Y[a:b] = z["Y"][:b - a]
+            found[a:b] = True
+            t += float(z["solver_s"]) * (b - a) / (int(z["b"]) - a)
+    args.dataset = args_dataset
+    m = int(np.argmin(found)) if not found.all() else n  # the leading complete runs
+    eln = np.array([[e.sig_nu, e.sig_nubar, e.alpha, e.seed] for e in elns[:m]])
+    np.savez_compressed(path, eln=eln, Y=Y[:m], solver_s=np.float64(t * m / max(1, found.sum())),
+                        meta=json.dumps(_meta(args, setup)))
+    log(json.dumps({"collected_runs": m, "file": path}))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--runs", type=int, default=2000)
+    ap.add_argument("--runs", type=int, default=10000)
     ap.add_argument("--sweep", type=int, default=256)
-    ap.add_argument("--nz", type=int, default=2048)
+    ap.add_argument("--nz", type=int, default=8192)
     ap.add_argument("--zhalf", type=float, default=600.0)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=8000)
+    ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--members", type=int, default=4)
     ap.add_argument("--bound", type=float, default=0.02)
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--out", default="")
+    ap.add_argument("--dataset", default="", help="collected dataset file (.npz)")
     ap.add_argument("--data-dir", default="", help="directory of dataset pieces data_<a>_<b>.npz")
     ap.add_argument("--gen", type=int, nargs=2, metavar=("A", "B"),
                     help="solve runs [A, B) of the sample into --data-dir and exit")
+    ap.add_argument("--collect", default="", help="gather the --data-dir pieces into this file")
     args = ap.parse_args(argv)
 
     setup = S.SolverSetup(nz=args.nz, z0=-args.zhalf, z1=args.zhalf, steps=args.steps)
@@ -129,12 +197,20 @@ def main(argv=None):
             ap.error("--gen A B needs --data-dir and 0 <= A < B")
         save_piece(args, setup, a, b)
         return 0
+    if args.collect:
+        if not args.data_dir:
+            ap.error("--collect needs --data-dir")
+        collect(args, setup, args.collect)
+        return 0
     elns = S.sample_elns(args.runs, args.seed)
     X = np.stack([S.features(e) for e in elns])
     Y, t_data, pieces = dataset(args, setup, elns)
     log(f"dataset: {args.runs} runs, solver {t_data:.1f} s")
     n_train = int(round(0.9 * args.runs))
-    Xtr, Ytr, Xte, Yte = X[:n_train], Y[:n_train], X[n_train:], Y[n_train:]
+    n_cal = (args.runs - n_train) // 2
+    Xtr, Ytr = X[:n_train], Y[:n_train]
+    Xca, Yca = X[n_train:n_train + n_cal], Y[n_train:n_train + n_cal]
+    Xte, Yte = X[n_train + n_cal:], Y[n_train + n_cal:]
 
     cfg = S.TrainConfig(steps=args.train_steps, members=args.members)
     t1 = time.perf_counter()
@@ -144,29 +220,38 @@ def main(argv=None):
         net, losses = S.train_member(Xtr, Ytr, cfg, seed=1000 + k, log=log)
         members.append(net)
         curves.append(losses)
-    S_ = S.Surrogate(members, bound=args.bound)
+    sweep_seed = args.seed + 1
+    S_ = S.Surrogate(members, bound=args.bound,
+                     meta={"dataset": _meta(args, setup), "train_runs": n_train,
+                           "calibration_runs": [n_train, n_train + n_cal],
+                           "sweep_seed": sweep_seed})
     t_train = time.perf_counter() - t1
-    cal = S_.calibrate(Xte, Yte)
+    cal = S_.calibrate(Xca, Yca)
     log(f"calibration: {cal}")
+    test = S_.evaluate(Xte, Yte)
+    log(f"test: {test}")
 
     # method without the network: predict the training-set mean profile
     base = Ytr.mean(axis=0)
-    base_err = float(np.abs(base[None, :] - Yte).mean())
+    base_err = float(S.run_errors(base[None, :], Yte).mean())
 
     # screened sweep on fresh configurations
-    fresh = S.sample_elns(args.sweep, args.seed + 1)
+    fresh = S.sample_elns(args.sweep, sweep_seed)
     Xs = np.stack([S.features(e) for e in fresh])
-    Ys, t_solver_all = S.solve(fresh, setup)
+    log("sweep: solver on every run")
+    Ys, t_solver_all = S.solve(fresh, setup, log=log)
     ta = time.perf_counter()
     pred, accept = S_.screen(Xs)
     t_infer = time.perf_counter() - ta
+    err_net = S.run_errors(pred, Ys)
     rejected = [fresh[i] for i in np.nonzero(~accept)[0]]
     if rejected:
+        log(f"sweep: solver on the {len(rejected)} rejected runs")
         Yr, t_fallback = S.solve(rejected, setup)
         pred[~accept] = Yr
     else:
         t_fallback = 0.0
-    err_sweep = np.abs(pred - Ys).mean(axis=1)
+    err_sweep = S.run_errors(pred, Ys)
     t_screened = t_infer + t_fallback
     out = {
         "metric": "surrogate-screened ELN sweep vs batched solver",
@@ -178,11 +263,14 @@ def main(argv=None):
         "fallback_s": t_fallback,
         "speedup": t_solver_all / t_screened if t_screened > 0 else None,
         "sweep_mean_abs_dPee": float(err_sweep.mean()),
-        "sweep_max_run_mean_abs_dPee": float(err_sweep.max()),
+        "sweep_max_accepted_run_abs_dPee": float(err_net[accept].max()) if accept.any() else None,
+        "sweep_max_run_abs_dPee_network_only": float(err_net.max()),
         "bound": args.bound,
-        "heldout": cal,
+        "calibration": cal,
+        "heldout": test,
         "no_network_baseline_heldout_mean_abs_dPee": base_err,
-        "dataset": {"runs": args.runs, "train": n_train, "heldout": args.runs - n_train,
+        "dataset": {"runs": args.runs, "train": n_train, "calibration": n_cal,
+                    "test": args.runs - n_train - n_cal,
                     "nz": setup.nz, "nvz": S.NV, "z": [setup.z0, setup.z1], "steps": setup.steps,
                     "avg_over_final_fraction": setup.avg_frac,
                     "sample_every_steps": setup.sample_every, "solver_s": t_data,
@@ -198,8 +286,7 @@ def main(argv=None):
         os.makedirs(args.out, exist_ok=True)
         with open(os.path.join(args.out, "surrogate_report.json"), "w") as f:
             f.write(line + "\n")
-        np.savez_compressed(os.path.join(args.out, "surrogate_weights.npz"),
-                            *[m.get_params() for m in members], tau=S_.tau)
+        S_.save(os.path.join(args.out, "surrogate_ensemble.npz"))
     return 0
 
 

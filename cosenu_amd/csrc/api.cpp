@@ -539,6 +539,13 @@ int capture_graph(hipStream_t s, int nsteps, Launch launch_step, hipGraphExec_t 
     (void)hipGraphDestroy(g);
     if (e != hipSuccess)
         return fail(COSENU_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    // move the graph's launch data to the device now, not at its first launch
+    e = hipGraphUpload(*out, s);
+    if (e != hipSuccess) {
+        (void)hipGraphExecDestroy(*out);
+        *out = nullptr;
+        return fail(COSENU_EHIP, std::string("hipGraphUpload: ") + hipGetErrorString(e));
+    }
     return 0;
 }
 

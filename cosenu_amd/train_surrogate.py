@@ -41,6 +41,7 @@ lacks are loaded from `--data-dir` pieces or solved.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -49,6 +50,8 @@ import time
 import numpy as np
 
 from . import surrogate as S
+
+REPLICA_SEED_XOR = 0x5DEECE66D  # phase seeds of the replica runs
 
 
 def log(msg):
@@ -75,6 +78,59 @@ def save_piece(args, setup, a, b):
         np.savez_compressed(path, X=X, Y=Y, a=a0, b=a1, solver_s=t,
                             meta=json.dumps(_meta(args, setup)))
         log(json.dumps({"piece": [a0, a1], "solver_s": t, "file": path}))
+
+
+def replica(e):
+    """The same ELN configuration with another phase seed."""
+    return dataclasses.replace(e, seed=e.seed ^ REPLICA_SEED_XOR)
+
+
+def save_replicas(args, setup, a, b):
+    """Runs [a, b) of the sample again with other phase seeds (rep_<a>_<b>.npz):
+    how far the solver's own targets move with the seed of the 1e-6
+    perturbation, which is not an input of the network."""
+    os.makedirs(args.data_dir, exist_ok=True)
+    sample = S.sample_elns(b, args.seed)
+    for a0 in range(a, b, setup.batch):
+        a1 = min(b, a0 + setup.batch)
+        Y, t = S.solve([replica(e) for e in sample[a0:a1]], setup, log=log)
+        path = os.path.join(args.data_dir, f"rep_{a0}_{a1}.npz")
+        np.savez_compressed(path, Y=Y, a=a0, b=a1, solver_s=t,
+                            meta=json.dumps(_meta(args, setup)))
+        log(json.dumps({"replicas": [a0, a1], "solver_s": t, "file": path}))
+
+
+def load_replicas(args, setup, n):
+    """Replica targets of runs [0, n) found in --data-dir or the dataset file:
+    (Y_rep with NaN rows where none, mask)."""
+    Yr = np.full((n, S.NV), np.nan, dtype=np.float32)
+    if args.dataset and os.path.exists(args.dataset):
+        with np.load(args.dataset, allow_pickle=False) as z:
+            if "Y_rep" in z.files and json.loads(str(z["meta"])) == _meta(args, setup):
+                m = min(n, len(z["Y_rep"]))
+                Yr[:m] = z["Y_rep"][:m]
+    if args.data_dir and os.path.isdir(args.data_dir):
+        for name in sorted(os.listdir(args.data_dir)):
+            if not (name.startswith("rep_") and name.endswith(".npz")):
+                continue
+            with np.load(os.path.join(args.data_dir, name), allow_pickle=False) as z:
+                if json.loads(str(z["meta"])) != _meta(args, setup):
+                    continue
+                a, b = int(z["a"]), min(int(z["b"]), n)
+                if a < b:
+                    Yr[a:b] = z["Y"][:b - a]
+    return Yr, ~np.isnan(Yr).any(axis=1)
+
+
+def seed_reproducibility(Y, Yr, have, bound):
+    """Per-run mean |dP_ee| between the solver's targets of one ELN
+    configuration under two phase seeds."""
+    if not have.any():
+        return None
+    d = S.run_errors(Y[have], Yr[have])
+    return {"runs": int(have.sum()), "mean": float(d.mean()), "median": float(np.median(d)),
+            "q90": float(np.quantile(d, 0.9)), "q95": float(np.quantile(d, 0.95)),
+            "max": float(d.max()), "fraction_above_bound": float((d > bound).mean())}
 
 
 def load_dataset_file(path, args, setup, elns):
@@ -162,12 +218,15 @@ def collect(args, setup, path):
             Y[a:b] = z["Y"][:b - a]
             found[a:b] = True
             t += float(z["solver_s"]) * (b - a) / (int(z["b"]) - a)
-    args.dataset = args_dataset
     m = int(np.argmin(found)) if not found.all() else n  # the leading complete runs
+    Yr, have_r = load_replicas(args, setup, m)
+    args.dataset = args_dataset
+    r = int(np.argmin(have_r)) if not have_r.all() else m  # leading runs with replicas
     eln = np.array([[e.sig_nu, e.sig_nubar, e.alpha, e.seed] for e in elns[:m]])
-    np.savez_compressed(path, eln=eln, Y=Y[:m], solver_s=np.float64(t * m / max(1, found.sum())),
+    np.savez_compressed(path, eln=eln, Y=Y[:m], Y_rep=Yr[:r],
+                        solver_s=np.float64(t * m / max(1, found.sum())),
                         meta=json.dumps(_meta(args, setup)))
-    log(json.dumps({"collected_runs": m, "file": path}))
+    log(json.dumps({"collected_runs": m, "replica_runs": r, "file": path}))
 
 
 def main(argv=None):
@@ -186,6 +245,8 @@ def main(argv=None):
     ap.add_argument("--data-dir", default="", help="directory of dataset pieces data_<a>_<b>.npz")
     ap.add_argument("--gen", type=int, nargs=2, metavar=("A", "B"),
                     help="solve runs [A, B) of the sample into --data-dir and exit")
+    ap.add_argument("--gen-replicas", type=int, nargs=2, metavar=("A", "B"),
+                    help="solve runs [A, B) again with other phase seeds into --data-dir and exit")
     ap.add_argument("--collect", default="", help="gather the --data-dir pieces into this file")
     args = ap.parse_args(argv)
 
@@ -197,6 +258,12 @@ def main(argv=None):
             ap.error("--gen A B needs --data-dir and 0 <= A < B")
         save_piece(args, setup, a, b)
         return 0
+    if args.gen_replicas:
+        a, b = args.gen_replicas
+        if not args.data_dir or not 0 <= a < b:
+            ap.error("--gen-replicas A B needs --data-dir and 0 <= A < B")
+        save_replicas(args, setup, a, b)
+        return 0
     if args.collect:
         if not args.data_dir:
             ap.error("--collect needs --data-dir")
@@ -206,6 +273,9 @@ def main(argv=None):
     X = np.stack([S.features(e) for e in elns])
     Y, t_data, pieces = dataset(args, setup, elns)
     log(f"dataset: {args.runs} runs, solver {t_data:.1f} s")
+    Yr, have_r = load_replicas(args, setup, args.runs)
+    seed_noise = seed_reproducibility(Y, Yr, have_r, args.bound)
+    log(f"seed reproducibility: {seed_noise}")
     n_train = int(round(0.9 * args.runs))
     n_cal = (args.runs - n_train) // 2
     Xtr, Ytr = X[:n_train], Y[:n_train]
@@ -269,6 +339,7 @@ def main(argv=None):
         "calibration": cal,
         "heldout": test,
         "no_network_baseline_heldout_mean_abs_dPee": base_err,
+        "solver_seed_reproducibility": seed_noise,
         "dataset": {"runs": args.runs, "train": n_train, "calibration": n_cal,
                     "test": args.runs - n_train - n_cal,
                     "nz": setup.nz, "nvz": S.NV, "z": [setup.z0, setup.z1], "steps": setup.steps,

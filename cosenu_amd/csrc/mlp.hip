@@ -19,12 +19,24 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <utility>
 #include <cstring>
 #include <string>
 #include <vector>
 
+// Scratch of cosenu_mlp_forward_noalloc: the BF16 copy of the input and the
+// hidden activations.  One copy per GPU in the code object, held until the
+// process ends; host code reaches it only through the device address that
+// hipGetSymbolAddress returns for the current device (resolved once per
+// device by cosenu_mlp_create, kept in g_scratch_ptr).
+constexpr size_t kNoallocScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
 namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr; // device address of `scratch`, by device ordinal
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -291,6 +303,30 @@ int backward(cosenu_mlp *m, int M)
     return 0;
 }
 
+// The device address of `scratch` on `device`, which must be current.
+int resolve_scratch(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device)
+        g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device])
+        return 0;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return mfail(COSENU_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+    return 0;
+}
+
+void *scratch_on(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
 int upload_batch(cosenu_mlp *m, const float *X, const float *T, int M)
 {
     if (M < 1 || M > m->maxB)
@@ -319,6 +355,11 @@ int cosenu_mlp_create(const int *dims, int nlayers, int max_batch, int device, c
         if (dims[l] < 1 || dims[l] > 4096)
             return mfail(COSENU_EINVAL, "cosenu_mlp_create: layer width out of range");
     MHIP(hipSetDevice(device));
+    {
+        const int rs = resolve_scratch(device); // for cosenu_mlp_forward_noalloc
+        if (rs)
+            return rs;
+    }
     cosenu_mlp *m = new cosenu_mlp();
     m->nl = nlayers;
     m->maxB = max_batch;
@@ -445,6 +486,55 @@ int cosenu_mlp_forward(cosenu_mlp *m, const float *X, int M, float *Y)
     MHIP(hipMemcpyAsync(Y, m->Z[m->nl - 1], (size_t)M * m->dims[m->nl] * 4, hipMemcpyDeviceToHost,
                         m->s));
     MHIP(hipStreamSynchronize(m->s));
+    return 0;
+}
+
+// cosenu_mlp_forward on device arrays without allocating: the same kernels
+// with the BF16 input and the hidden activations in `scratch` (laid out as
+// X16 [M x dims[0]], then two activation buffers [M x widest hidden layer]
+// used in turn, each rounded up to 256 bytes); the last layer writes Y.
+// Enqueues only.
+int cosenu_mlp_forward_noalloc(cosenu_mlp *m, const float *X, int M, float *Y, void *stream)
+{
+    if (!m || !X || !Y)
+        return mfail(COSENU_EINVAL, "cosenu_mlp_forward_noalloc: null argument");
+    if (M < 1)
+        return mfail(COSENU_EINVAL, "cosenu_mlp_forward_noalloc: M must be at least 1");
+    int hid = 0;
+    for (int l = 1; l < m->nl; l++)
+        hid = std::max(hid, m->dims[l]);
+    const size_t row = align256((size_t)m->dims[0] * 2) + 2 * align256((size_t)hid * 2);
+    const size_t need = align256((size_t)M * m->dims[0] * 2) + 2 * align256((size_t)M * hid * 2);
+    if (need > kNoallocScratchBytes)
+        return mfail(COSENU_EINVAL, "cosenu_mlp_forward_noalloc: " + std::to_string(M) +
+                                        " rows need " + std::to_string(need) +
+                                        " bytes of scratch; it holds " +
+                                        std::to_string(kNoallocScratchBytes) + " (about " +
+                                        std::to_string(kNoallocScratchBytes / row) +
+                                        " rows of this network)");
+    MHIP(hipSetDevice(m->device));
+    unsigned char *base = static_cast<unsigned char *>(scratch_on(m->device));
+    if (!base)
+        return mfail(COSENU_ESTATE, "cosenu_mlp_forward_noalloc: scratch not resolved on this device");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : m->s;
+    bf16_t *x16 = reinterpret_cast<bf16_t *>(base);
+    bf16_t *act[2];
+    act[0] = reinterpret_cast<bf16_t *>(base + align256((size_t)M * m->dims[0] * 2));
+    act[1] = reinterpret_cast<bf16_t *>(reinterpret_cast<unsigned char *>(act[0]) +
+                                        align256((size_t)M * hid * 2));
+    const long nx = (long)M * m->dims[0];
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(nx, 256)), dim3(256), 0, s, X, nx, x16);
+    MHIP(hipGetLastError());
+    const bf16_t *in = x16;
+    for (int l = 0; l < m->nl; l++) {
+        const int K = m->dims[l], N = m->dims[l + 1];
+        const bool last = l == m->nl - 1;
+        int rc = gemm(s, in, K, 1, m->W16[l], N, 1, M, N, K, m->b[l], last ? Y : nullptr, N,
+                      last ? nullptr : act[l & 1], N, EPI_GELU);
+        if (rc)
+            return rc;
+        in = act[l & 1];
+    }
     return 0;
 }
 

@@ -276,6 +276,20 @@ long long cosenu_mlp_num_params(cosenu_mlp *m);
 int cosenu_mlp_set_params(cosenu_mlp *m, const float *flat);
 int cosenu_mlp_get_params(cosenu_mlp *m, float *flat);
 int cosenu_mlp_forward(cosenu_mlp *m, const float *X, int M, float *Y);
+/* Inference that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops).  X (M x dims[0]) and Y (M x dims[nlayers]) are
+ * FP32 DEVICE arrays; the BF16 copy of X and the hidden activations live in
+ * one 16 MiB array of the library's code object per GPU, so M x (dims[0] +
+ * 2 x widest hidden layer) x 2 bytes (each part rounded up to 256 bytes)
+ * must fit (about 14,000 rows of the 67-256-256-256-64 network), else
+ * COSENU_EINVAL and a message.  Enqueues on `stream` (a hipStream_t; NULL =
+ * the network's own stream, ordered after its training steps: then
+ * cosenu_mlp_sync waits for it) and returns; Y equals cosenu_mlp_forward's
+ * output bit for bit.  Every call on one GPU uses that GPU's one array:
+ * calls that overlap in time on one GPU (other streams, other networks)
+ * race. */
+int cosenu_mlp_forward_noalloc(cosenu_mlp *m, const float *X, int M, float *Y, void *stream);
 /* mean squared error over all M x dims[nlayers] outputs and its gradient */
 int cosenu_mlp_gradients(cosenu_mlp *m, const float *X, const float *T, int M, float *grad,
                          float *loss);

@@ -364,9 +364,12 @@ struct cosenu_ctx {
     std::vector<int> ev_stage;
     double ms[4] = {0, 0, 0, 0};
     long long cnt[4] = {0, 0, 0, 0};
-    hipGraphExec_t graph = nullptr;
-    int graph_steps = 0;
-    const double *graph_y = nullptr; // the state buffer the graph was captured with
+    // captured step graphs, one per state buffer they start from (the two-
+    // and four-stage launches alternate two buffers): graph[k] starts from
+    // graph_y[k] and runs graph_steps[k] steps
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_steps[2] = {0, 0};
+    const double *graph_y[2] = {nullptr, nullptr};
     hipEvent_t snap_ready = nullptr, snap_done = nullptr;
     bool snap_pending = false;
 };
@@ -397,8 +400,9 @@ void release(cosenu_ctx *c)
             (void)hipFree(b);
     for (hipEvent_t e : c->ev)
         (void)hipEventDestroy(e);
-    if (c->graph)
-        (void)hipGraphExecDestroy(c->graph);
+    for (hipGraphExec_t g : c->graph)
+        if (g)
+            (void)hipGraphExecDestroy(g);
     if (c->snap_ready)
         (void)hipEventDestroy(c->snap_ready);
     if (c->snap_done)
@@ -560,19 +564,26 @@ int graph_steps(const cosenu_ctx *c)
     return gsteps;
 }
 
-// The graph of gsteps steps starting from the current state buffer.
-int ensure_graph(cosenu_ctx *c, int gsteps)
+// The graph of gsteps steps starting from the current state buffer
+// (captured if there is none); returns it in *g.
+int ensure_graph(cosenu_ctx *c, int gsteps, hipGraphExec_t *g)
 {
-    if (c->graph && c->graph_steps == gsteps && c->graph_y == c->y)
+    int k = c->graph_y[0] == c->y ? 0 : c->graph_y[1] == c->y ? 1 : -1;
+    if (k >= 0 && c->graph[k] && c->graph_steps[k] == gsteps) {
+        *g = c->graph[k];
         return 0;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
     }
-    c->graph_y = c->y;
+    if (k < 0) // a slot not holding the other buffer's graph
+        k = (c->graph[0] && c->graph_y[0] == c->acc && swaps_state(c)) ? 1 : 0;
+    if (c->graph[k]) {
+        (void)hipGraphExecDestroy(c->graph[k]);
+        c->graph[k] = nullptr;
+    }
+    c->graph_y[k] = c->y;
     auto one = [c]() -> int { return enqueue_step(c, c->stream); };
-    RC_TRY(capture_graph(c->stream, gsteps, one, &c->graph));
-    c->graph_steps = gsteps;
+    RC_TRY(capture_graph(c->stream, gsteps, one, &c->graph[k]));
+    c->graph_steps[k] = gsteps;
+    *g = c->graph[k];
     return 0;
 }
 
@@ -874,7 +885,17 @@ int cosenu_prepare(cosenu_ctx *c)
     const int gsteps = graph_steps(c);
     if (c->timing || gsteps <= 0)
         return 0;
-    return ensure_graph(c, gsteps);
+    hipGraphExec_t g = nullptr;
+    RC_TRY(ensure_graph(c, gsteps, &g));
+    if (swaps_state(c)) {
+        // the graph that starts from the other buffer (after an odd number of
+        // steps); capture runs nothing, the swap is bookkeeping only
+        swap_state(c);
+        const int rc = ensure_graph(c, gsteps, &g);
+        swap_state(c);
+        RC_TRY(rc);
+    }
+    return 0;
 }
 
 int cosenu_step(cosenu_ctx *c, int nsteps)
@@ -889,9 +910,10 @@ int cosenu_step(cosenu_ctx *c, int nsteps)
     const int gsteps = graph_steps(c);
     int done = 0;
     if (!c->timing && gsteps > 0 && nsteps >= gsteps) {
-        RC_TRY(ensure_graph(c, gsteps));
+        hipGraphExec_t g = nullptr;
+        RC_TRY(ensure_graph(c, gsteps, &g));
         for (; done + gsteps <= nsteps; done += gsteps)
-            HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+            HIP_TRY(hipGraphLaunch(g, c->stream));
     }
     for (; done < nsteps; done++) {
         RC_TRY(enqueue_step(c, c->stream));

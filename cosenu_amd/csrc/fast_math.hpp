@@ -119,13 +119,15 @@ __device__ inline void swap16(double &a, double &b)
     b = __hiloint2double((int)hi[1], (int)lo[1]);
 }
 
-// x from another lane of the same row of 16 (DPP control CTRL).
+// x from another lane of the same row of 16 (DPP control CTRL).  The move is
+// one 64-bit DPP instruction for the row broadcasts (row_newbcast, the one
+// control gfx950 takes on 64-bit operands) and two 32-bit ones otherwise.
 template <int CTRL>
 __device__ inline double dpp(double x)
 {
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xf, 0xf, true);
-    return __hiloint2double(hi, lo);
+    long v = __builtin_bit_cast(long, x);
+    long r = __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, r);
 }
 
 // Sum of 6 per-lane values m[0..5] over the 64 lanes.  Each level halves the

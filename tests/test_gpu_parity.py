@@ -139,6 +139,29 @@ def test_fast_deterministic_and_graph_equals_eager(gpu, monkeypatch):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
+@pytest.mark.parametrize("scheme,variant", [(O.FV, "4"), (O.FD, "5"), (O.FV, "3")])
+def test_prepared_graphs_equal_eager_steps(gpu, monkeypatch, scheme, variant):
+    """cosenu_prepare captures the step graph for both state buffers the two-
+    and four-stage launches alternate; after an odd number of steps the
+    graph of the other buffer is replayed.  Results equal steps launched one
+    by one, bit for bit."""
+    monkeypatch.setenv("COSENU_FAST_VARIANT", variant)
+    po, pc = _pair(scheme, O.PERIODIC, O.VAC | O.COLL, 256, 64, C.FAST)
+    _, _, s = _state(po, 23)
+    outs = []
+    for gsteps in ("16", "0"):
+        monkeypatch.setenv("COSENU_GRAPH_STEPS", gsteps)
+        with C.Simulation(pc) as sim:
+            assert sim.info()["fast_variant"] == int(variant)
+            sim.upload(s)
+            sim.prepare()
+            sim.step(3)   # odd: the state now starts in the other buffer
+            sim.step(16)  # one graph replay from that buffer
+            sim.step(17)  # one replay from the first buffer, one step alone
+            outs.append(sim.download())
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_layout_round_trips(gpu):
     _, pc = _pair(O.FV, O.PERIODIC, O.VAC | O.COLL, 72, 40, C.STRICT)
     rng = np.random.default_rng(3)

@@ -289,10 +289,11 @@ def sweep_speedup(C, window: int, log):
     averaged over the final 20% of the steps, sampled every 20) in one batch;
     `window` batch steps are timed after a graph-capturing warm-up and the
     8000-step time is extrapolated linearly, plus the 80 samples of the
-    average (one timed).  Screened: features, ensemble inference on the
-    device, then the rejected runs alone through the same solver schedule
-    (timed the same way).  The accuracy of the screened result is the
-    committed record of the full pipeline run, which solved all 256 runs."""
+    average (one timed).  Screened, for each of the ensemble's two gates
+    (sweep-mean and per-run): features, ensemble inference on the device,
+    then the rejected runs alone through the same solver schedule (timed the
+    same way).  The accuracy of the screened results is the committed record
+    of the full pipeline run, which solved all 256 runs."""
     import numpy as np
     from cosenu_amd import surrogate as S
     if not os.path.exists(SURROGATE_FILE):
@@ -326,30 +327,35 @@ def sweep_speedup(C, window: int, log):
 
     log("sweep: solver-only timing")
     t_solver, step_all, samp_all = solver_seconds(elns)
-    log("sweep: screened timing")
-    t0 = time.perf_counter()
-    X = np.stack([S.features(e) for e in elns])
-    t_feat = time.perf_counter() - t0
-    surr.predict(X[:8])  # first call loads the kernels
-    t0 = time.perf_counter()
-    _, accept = surr.screen(X)
-    t_inf = time.perf_counter() - t0
-    rejected = [e for e, ok in zip(elns, accept) if not ok]
-    t_fb, step_fb, samp_fb = solver_seconds(rejected) if rejected else (0.0, 0.0, 0.0)
-    t_screened = t_feat + t_inf + t_fb
     out = {"runs": 256, "grid": "nz=8192 on [-600,600], nvz=64, 8000 steps, FV, P_ee(v) "
                                  "averaged over steps 6400-8000 every 20",
            "window_steps": window, "solver_s_per_step": step_all,
            "solver_s_per_sample": samp_all, "solver_only_s": t_solver,
-           "features_s": t_feat, "inference_s": t_inf, "fallback_runs": len(rejected),
-           "fallback_s": t_fb, "screened_s": t_screened, "speedup": t_solver / t_screened,
-           "tau": surr.tau, "members": len(surr.members)}
+           "members": len(surr.members), "bound": surr.bound}
+    t0 = time.perf_counter()
+    X = np.stack([S.features(e) for e in elns])
+    t_feat = time.perf_counter() - t0
+    surr.predict(X[:8])  # first call loads the kernels
+    for gate, key in (("mean", "sweep_mean_gate"), ("run", "per_run_gate")):
+        log(f"sweep: screened timing, {gate} gate")
+        t0 = time.perf_counter()
+        _, accept = surr.screen(X, gate=gate)
+        t_inf = time.perf_counter() - t0
+        rejected = [e for e, ok in zip(elns, accept) if not ok]
+        t_fb = solver_seconds(rejected)[0] if rejected else 0.0
+        t_screened = t_feat + t_inf + t_fb
+        out[key] = {"tau": surr.tau if gate == "run" else surr.tau_mean,
+                    "features_s": t_feat, "inference_s": t_inf, "fallback_runs": len(rejected),
+                    "fallback_s": t_fb, "screened_s": t_screened,
+                    "speedup": t_solver / t_screened}
+    # the headline figure: the gate that holds the bound on the sweep's mean
+    out["speedup"] = out["sweep_mean_gate"]["speedup"]
     if os.path.exists(SURROGATE_RECORD):
         with open(SURROGATE_RECORD) as fh:
             rec = json.load(fh)
         out["accuracy_from_pipeline_record"] = {
-            k: rec.get(k) for k in ("sweep_mean_abs_dPee", "sweep_max_accepted_run_abs_dPee",
-                                    "heldout", "bound", "fallback_runs", "dataset")}
+            k: rec.get(k) for k in ("per_run_gate", "sweep_mean_gate", "heldout", "bound",
+                                    "solver_seed_reproducibility", "dataset")}
     for m in surr.members:
         m.close()
     return out

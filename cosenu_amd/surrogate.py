@@ -16,10 +16,12 @@ over many ELN configurations can skip most solver runs.
   the hand-written kernels of csrc/mlp.hip (cosenu_mlp_*).
 * Fallback: an ensemble of independently initialised networks; the mean
   over bins of the ensemble standard deviation is the per-run error
-  indicator.  Its acceptance threshold is calibrated on solver runs held out
-  of training so that no calibration run whose own error (mean over bins of
-  |dP_ee|) exceeds the bound (2%) is accepted, and measured on a second
-  held-out set; every rejected run is run by the solver.
+  indicator.  Two acceptance thresholds are calibrated on solver runs held
+  out of training and measured on a second held-out set: a per-run gate (no
+  calibration run whose own error, the mean over bins of |dP_ee|, exceeds
+  the bound (2%) is accepted) and a sweep-mean gate (the screened sweep's
+  mean error stays within the bound).  Every rejected run is run by the
+  solver.
 """
 from __future__ import annotations
 
@@ -270,19 +272,28 @@ def run_errors(pred, Y):
 
 
 class Surrogate:
-    """Ensemble of trained networks with a per-run acceptance gate.
+    """Ensemble of trained networks with two acceptance gates.
 
-    The gate is the ensemble spread (mean over bins of the members' standard
-    deviation).  `calibrate` sets its threshold tau on solver runs the
-    networks were not trained on, so that no calibration run whose error
-    exceeds the bound is accepted; `evaluate` then measures the gate on
-    further unseen runs (accept rate, largest accepted error, recall of the
-    runs above the bound).  Every rejected run goes to the solver."""
+    Both gate on the ensemble spread (mean over bins of the members'
+    standard deviation) with thresholds set by `calibrate` on solver runs the
+    networks were not trained on; every rejected run goes to the solver.
+
+    * per-run gate (tau): just below the smallest spread of a calibration
+      run whose own error (mean over bins of |dP_ee|) exceeds the bound, so
+      that no calibration run above the bound is accepted;
+    * sweep-mean gate (tau_mean): the largest threshold at which the
+      screened calibration sweep (accepted runs at their network error,
+      rejected runs at the solver's, i.e. zero) keeps its mean error within
+      the bound.
+
+    `evaluate` measures both on further unseen runs (accept rate, largest
+    accepted error, recall of the runs above the bound, screened mean)."""
 
     def __init__(self, members: list, bound: float = 0.02, meta: dict | None = None):
         self.members = members
         self.bound = bound
-        self.tau = -1.0  # acceptance threshold on the ensemble spread
+        self.tau = -1.0       # per-run gate threshold on the ensemble spread
+        self.tau_mean = -1.0  # sweep-mean gate threshold
         self.meta = dict(meta or {})
 
     def predict(self, X):
@@ -290,8 +301,6 @@ class Surrogate:
         return P.mean(axis=0), P.std(axis=0).mean(axis=1)
 
     def calibrate(self, X, Y):
-        """tau just below the smallest spread of a calibration run whose error
-        exceeds the bound (every run's spread if none does)."""
         mean, spread = self.predict(X)
         err = run_errors(mean, Y)
         bad = err > self.bound
@@ -299,39 +308,48 @@ class Surrogate:
             self.tau = float(np.nextafter(spread[bad].min(), -np.inf))
         else:
             self.tau = float(spread.max())
-        acc = spread <= self.tau
-        return {"runs": int(len(err)), "above_bound": int(bad.sum()),
-                "accept_rate": float(acc.mean()),
-                "max_accepted_run_err": float(err[acc].max()) if acc.any() else None,
-                "mean_err_all": float(err.mean()), "tau": self.tau}
+        order = np.argsort(spread, kind="stable")
+        screened_mean = np.cumsum(err[order]) / len(err)  # accept the k least spread
+        ok = np.nonzero(screened_mean <= self.bound)[0]
+        self.tau_mean = float(spread[order][ok[-1]]) if ok.size else -1.0
+        out = {"runs": int(len(err)), "above_bound": int(bad.sum()),
+               "mean_err_all": float(err.mean()), "tau": self.tau, "tau_mean": self.tau_mean}
+        for g, t in (("run", self.tau), ("mean", self.tau_mean)):
+            acc = spread <= t
+            out[f"{g}_gate_accept_rate"] = float(acc.mean())
+            out[f"{g}_gate_max_accepted_run_err"] = float(err[acc].max()) if acc.any() else None
+        return out
 
     def evaluate(self, X, Y):
-        """The gate on unseen runs: rejected runs take the solver's values."""
+        """Both gates on unseen runs: rejected runs take the solver's values."""
         mean, spread = self.predict(X)
         err = run_errors(mean, Y)
-        acc = spread <= self.tau
         bad = err > self.bound
-        screened = np.where(acc, err, 0.0)
-        return {"runs": int(len(err)), "accept_rate": float(acc.mean()),
+        out = {"runs": int(len(err)), "above_bound": int(bad.sum()),
+               "mean_err_network_only": float(err.mean())}
+        for g, t in (("run", self.tau), ("mean", self.tau_mean)):
+            acc = spread <= t
+            out[f"{g}_gate"] = {
+                "accept_rate": float(acc.mean()),
                 "max_accepted_run_err": float(err[acc].max()) if acc.any() else None,
                 "mean_accepted_run_err": float(err[acc].mean()) if acc.any() else None,
-                "above_bound": int(bad.sum()),
                 "recall_of_above_bound": float((bad & ~acc).sum() / bad.sum()) if bad.any() else None,
                 "accepted_above_bound": int((bad & acc).sum()),
-                "mean_err_network_only": float(err.mean()),
-                "mean_err_screened": float(screened.mean())}
+                "mean_err_screened": float(np.where(acc, err, 0.0).mean())}
+        return out
 
-    def screen(self, X):
+    def screen(self, X, gate: str = "run"):
+        """Network prediction and the accept mask of a gate ("run" or "mean")."""
         mean, spread = self.predict(X)
-        return mean, spread <= self.tau
+        return mean, spread <= (self.tau if gate == "run" else self.tau_mean)
 
     def save(self, path: str):
         """Ensemble, threshold and metadata in one .npz (no pickles)."""
         import json
         params = np.stack([m.get_params() for m in self.members])
         np.savez_compressed(path, params=params, dims=np.array(self.members[0].dims),
-                            tau=np.float64(self.tau), bound=np.float64(self.bound),
-                            meta=json.dumps(self.meta))
+                            tau=np.float64(self.tau), tau_mean=np.float64(self.tau_mean),
+                            bound=np.float64(self.bound), meta=json.dumps(self.meta))
 
     @classmethod
     def load(cls, path: str, max_batch: int = 4096, device: int = 0) -> "Surrogate":
@@ -345,4 +363,5 @@ class Surrogate:
                 members.append(m)
             s = cls(members, bound=float(z["bound"]), meta=json.loads(str(z["meta"])))
             s.tau = float(z["tau"])
+            s.tau_mean = float(z["tau_mean"])
         return s

@@ -34,21 +34,25 @@ def test_ensemble_beats_no_network_and_fallback_holds_the_bound(gpu, small):
     members = [S.train_member(Xtr, Ytr, cfg, seed=50 + k)[0] for k in range(cfg.members)]
     sur = S.Surrogate(members, bound=0.02)
     cal = sur.calibrate(Xca, Yca)
-    # by construction no calibration run above the bound is accepted
-    assert cal["max_accepted_run_err"] is None or cal["max_accepted_run_err"] <= 0.02
+    # by construction no calibration run above the bound passes the per-run gate
+    m = cal["run_gate_max_accepted_run_err"]
+    assert m is None or m <= 0.02
     test = sur.evaluate(Xte, Yte)
     base_err = float(S.run_errors(Ytr.mean(axis=0)[None, :], Yte).mean())
     assert test["mean_err_network_only"] < 0.7 * base_err, (test, base_err)
     # screened prediction: accepted runs from the network, the rest from the solver
-    pred, accept = sur.screen(Xte)
-    pred = pred.copy()
-    if (~accept).any():
-        idx = np.nonzero(~accept)[0]
-        pred[idx], _ = S.solve([elns[192 + i] for i in idx], setup)
-    err = S.run_errors(pred, Yte)
-    assert err[~accept].max(initial=0.0) == 0.0  # the solver's own values
-    assert err.mean() == pytest.approx(test["mean_err_screened"], abs=1e-6)
-    assert err.mean() <= 0.02
+    for gate in ("run", "mean"):
+        pred, accept = sur.screen(Xte, gate=gate)
+        pred = pred.copy()
+        if (~accept).any():
+            idx = np.nonzero(~accept)[0]
+            pred[idx], _ = S.solve([elns[192 + i] for i in idx], setup)
+        err = S.run_errors(pred, Yte)
+        assert err[~accept].max(initial=0.0) == 0.0  # the solver's own values
+        assert err.mean() == pytest.approx(test[f"{gate}_gate"]["mean_err_screened"], abs=1e-6)
+        assert err.mean() <= test["mean_err_network_only"] + 1e-9
+        if gate == "run":
+            assert err.mean() <= 0.02
     # a threshold below every spread sends every run to the solver
     sur.tau = -1.0
     _, acc2 = sur.screen(Xte)

@@ -113,12 +113,24 @@ def test_gate_rejects_every_calibration_run_above_the_bound():
     cal = sur.calibrate(X, truth)
     # runs 2 and 3 exceed the bound; the smaller of their spreads is 0.003
     assert sur.tau < 0.003 and sur.tau > 0.0029999
-    assert cal["above_bound"] == 2 and cal["max_accepted_run_err"] <= 0.02
-    assert cal["accept_rate"] == 2 / 6
+    assert cal["above_bound"] == 2 and cal["run_gate_max_accepted_run_err"] <= 0.02
+    assert cal["run_gate_accept_rate"] == 2 / 6
     ev = sur.evaluate(X, truth)
-    assert ev["accepted_above_bound"] == 0 and ev["recall_of_above_bound"] == 1.0
-    assert np.isclose(ev["mean_err_screened"], (0.005 + 0.010) / 6)
+    g = ev["run_gate"]
+    assert g["accepted_above_bound"] == 0 and g["recall_of_above_bound"] == 1.0
+    assert np.isclose(g["mean_err_screened"], (0.005 + 0.010) / 6)
+    # sweep-mean gate: runs accepted in order of spread while the screened
+    # mean (rejected runs exact) stays within the bound -- here all of them,
+    # 0.101 / 6 = 0.0168
+    assert np.isclose(sur.tau_mean, 0.020) and cal["mean_gate_accept_rate"] == 1.0
+    assert np.isclose(ev["mean_gate"]["mean_err_screened"], 0.101 / 6)
+    _, acc = sur.screen(X, gate="mean")
+    assert acc.all()
+    # a tighter bound: the three least-spread runs (0.045 / 6 <= 0.009)
+    sur3 = S.Surrogate([_Member(a), _Member(b)], bound=0.009)
+    sur3.calibrate(X, truth)
+    assert np.isclose(sur3.tau_mean, 0.003)
     # no run above the bound: every calibration spread is accepted
     sur2 = S.Surrogate([_Member(truth + 0.001), _Member(truth - 0.001)], bound=0.02)
     cal2 = sur2.calibrate(X, truth)
-    assert cal2["accept_rate"] == 1.0 and cal2["above_bound"] == 0
+    assert cal2["run_gate_accept_rate"] == 1.0 and cal2["above_bound"] == 0

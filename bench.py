@@ -232,7 +232,8 @@ def measure(C, name, scheme, steps, warmup, timing_steps):
     traffic = traffic_rec.get(dom) if traffic_rec else None
     res = {
         "workload": f"{wl.name}: {scheme.upper()} RK4, nz={p.nz} on [{p.z0:g},{p.z1:g}], "
-                    f"nvz={p.nvz}, CFL={p.CFL}, periodic, mu=1, omega=0, FAST mode",
+                    f"nvz={p.nvz}, CFL={p.CFL}, {'open' if p.bc else 'periodic'}, mu=1, omega=0, "
+                    "FAST mode",
         "cells": cells, "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * el / steps,
         "value": cells * steps / el, "unit": "cell-updates/s",
         "frac_at_1088B_per_cell_update": cells * steps / el * BYTES_PER_CELL_STEP_DEF / 1e9
@@ -370,7 +371,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", action="store_true",
                     help="the headline workload only (no other workloads, no sweep)")
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c4", "wide"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c1_open", "c4", "wide", "wide_open"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every GPU box here has one, and the library "

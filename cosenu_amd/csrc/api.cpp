@@ -211,12 +211,10 @@ int validate(const cosenu_params *p)
         return fail(COSENU_EINVAL, "mode must be COSENU_STRICT or COSENU_FAST");
     if (!(p->z1 > p->z0) || !(p->v1 > p->v0) || !(p->CFL > 0))
         return fail(COSENU_EINVAL, "need z1 > z0, v1 > v0, CFL > 0");
-    const int VP = padded_bins(p->nvz);
-    // nvz > 256: the barrier-free kernel alone (periodic; the groups of 32
-    // bins are checked for one sign of v when the context is created)
-    if (p->mode == COSENU_FAST && !fast_supported(VP, p->bc) &&
-        !(VP > 256 && p->bc == COSENU_PERIODIC))
-        return fail(COSENU_ENOTSUP, "FAST mode with nvz > 256 needs periodic boundaries");
+    // FAST mode takes every grid: rows of up to 256 bins on the team kernels
+    // (either boundary), wider rows on the barrier-free kernel (its general
+    // form for groups of 32 bins that straddle v = 0, padding bins and the
+    // open boundary)
     return 0;
 }
 
@@ -309,9 +307,9 @@ void tune_store(int VP, int scheme, int nz, TuneVal v)
 // as many teams (one wavefront per SIMD) finish sooner: below 8 rows per
 // resident team for FV, 3 for FD.  (nz = 1024, nvz = 64: 512 teams of 2
 // rows, 63 us per FV step against 90 us with 128 teams of 8 rows.)
-int range_teams(int nz, int VP, int scheme, int bc, int variant, int mult)
+int range_teams(int nz, int VP, int scheme, int bc, int variant, int mult, bool general)
 {
-    long t = variant == 3   ? (long)stream_resident_waves(scheme) / (VP / 32)
+    long t = variant == 3   ? (long)stream_resident_waves(scheme, general) / (VP / 32)
              : variant == 4 ? (long)pair_resident_teams(VP, scheme)
              : variant == 5 ? (long)quad_resident_teams(VP)
                             : (long)fast_resident_teams(VP, scheme, bc);
@@ -356,6 +354,7 @@ struct cosenu_ctx {
     int z_chunk = 0, variant = 2;
     int nteams = 0; // > 0: contiguous-range mode with this many teams
     bool stream_ok = false; // the barrier-free kernel (variant 3) applies
+    bool stream_general = false; // ... in its general form (nvz > 256 only)
     bool uniform = false;   // nvz == VP, every group of 32 bins of one sign of v
     double *mom[4] = {nullptr, nullptr, nullptr, nullptr}; // its moments of y, A, B, acc
     View view;
@@ -448,7 +447,7 @@ hipError_t launch_stage(const cosenu_ctx *c, int stage, hipStream_t s)
 {
     StageArgs a = stage_args(c, stage);
     if (c->p.mode == COSENU_FAST && c->variant == 3)
-        return launch_stage_stream(a, stage, c->z_chunk, c->nteams, s);
+        return launch_stage_stream(a, stage, c->z_chunk, c->nteams, c->stream_general, s);
     if (c->p.mode == COSENU_FAST)
         return launch_stage_fast(a, stage, c->z_chunk, c->nteams, s);
     return launch_stage_strict(a, stage, s);
@@ -615,9 +614,10 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
                  env_int("COSENU_UNIFORM", 1) != 0;
     c->stream_ok = p->mode == COSENU_FAST && c->uniform && p->bc == COSENU_PERIODIC;
     if (p->mode == COSENU_FAST && c->VP > 256 && !c->stream_ok) {
-        delete c;
-        return fail(COSENU_ENOTSUP, "FAST mode with nvz > 256 needs nvz a multiple of 64 and "
-                                    "every group of 32 velocity bins on one side of v = 0");
+        // groups that straddle v = 0, padding bins or the open boundary: the
+        // general form of the barrier-free kernel
+        c->stream_ok = true;
+        c->stream_general = true;
     }
     if (p->mode == COSENU_FAST) {
         // z_chunk < 0 in the tune table = contiguous-range mode with that many
@@ -627,7 +627,8 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
         if (tune_lookup(c->VP, p->scheme, p->nz, &tv)) {
             c->variant = tv.variant;
             chunk = tv.chunk;
-        } else if (c->stream_ok && c->nz >= 16 && (long)c->nz * (c->VP / 32) <= 2048) {
+        } else if (c->stream_ok && !c->stream_general && c->nz >= 16 &&
+                   (long)c->nz * (c->VP / 32) <= 2048) {
             // small problems (at most 2 wavefronts per SIMD in all): the
             // barrier-free kernel in chunks of 2 rows (nz = 1024, nvz = 64:
             // 27 us per FV step against 45 us for two-stage ranges; FD 20 us
@@ -662,13 +663,12 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
         if (c->variant == 3 && !c->stream_ok)
             c->variant = 2; // shapes the barrier-free kernel does not take
         if (c->variant == 5 && (!quad_supported(c->VP, p->bc, p->scheme, p->flags) || c->nz < 16))
-            c->variant = 4; // FV, 5th-order dissipation, open boundary, nvz > 256
+            c->variant = 4; // FV, 5th-order dissipation, open boundary
         if (c->variant == 4 && (!pair_supported(c->VP, p->bc) || c->nz < 16))
             c->variant = 2;
-        if (p->bc == COSENU_OPEN)
-            c->variant = 2; // the open boundary exists in the split-lane kernel only
         if (chunk < 0) {
-            c->nteams = range_teams(c->nz, c->VP, p->scheme, p->bc, c->variant, -chunk);
+            c->nteams = range_teams(c->nz, c->VP, p->scheme, p->bc, c->variant, -chunk,
+                                   c->stream_general);
             c->z_chunk = (c->nz + c->nteams - 1) / c->nteams;
         } else {
             if (chunk <= 0 || chunk > c->nz || c->nz % chunk)
@@ -906,7 +906,7 @@ int cosenu_step(cosenu_ctx *c, int nsteps)
     if (nsteps > 0 && c->p.mode == COSENU_FAST && c->variant == 3)
         // the state may have been replaced since the last step: its moments
         // are rebuilt here, after that every stage 4 leaves them current
-        HIP_TRY(launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->VP, c->stream));
+        HIP_TRY(launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->nvz, c->VP, c->stream));
     const int gsteps = graph_steps(c);
     int done = 0;
     if (!c->timing && gsteps > 0 && nsteps >= gsteps) {
@@ -945,11 +945,11 @@ int cosenu_rhs(cosenu_ctx *c, double *const out[8], ptrdiff_t stride, int off)
     HIP_TRY(hipSetDevice(c->p.device));
     RhsArgs r = rhs_args(c, c->y, c->B);
     if (c->p.mode == COSENU_FAST && c->variant == 3) {
-        HIP_TRY(launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->VP, c->stream));
+        HIP_TRY(launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->nvz, c->VP, c->stream));
         StageArgs a = stage_args(c, 1);
         a.yout = c->B;
         a.mom_out = nullptr;
-        HIP_TRY(launch_stage_stream(a, 0, c->z_chunk, c->nteams, c->stream));
+        HIP_TRY(launch_stage_stream(a, 0, c->z_chunk, c->nteams, c->stream_general, c->stream));
     } else if (c->p.mode == COSENU_FAST) { // variants 4, 5 evaluate single RHSs with variant 2
         HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->nteams, c->stream));
     } else {
@@ -1059,7 +1059,7 @@ const char *cosenu_stage_kernel_name(cosenu_ctx *c, int stage)
     if (!c)
         return "";
     if (c->p.mode == COSENU_FAST && c->variant == 3)
-        return stage_stream_kernel_name(stage);
+        return stage_stream_kernel_name(stage, c->stream_general);
     if (four_stage_launch(c))
         return stage == 0 ? stage_quad_kernel_name() : "";
     if (two_stage_launches(c))
@@ -1100,8 +1100,8 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
                 (var == 5 && (!quad_supported(c->VP, q.bc, q.scheme, q.flags) || c->nz < 16)))
                 continue;
             c->variant = var;
-            if (var == 3 && launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->VP,
-                                                  c->stream) != hipSuccess) {
+            if (var == 3 && launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->nvz,
+                                                  c->VP, c->stream) != hipSuccess) {
                 rc = fail(COSENU_EHIP, "cosenu_autotune: launch failed");
                 break;
             }
@@ -1109,7 +1109,7 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
                 c->z_chunk = L;
                 c->nteams = 0;
             } else {
-                c->nteams = range_teams(c->nz, c->VP, q.scheme, q.bc, var, -L);
+                c->nteams = range_teams(c->nz, c->VP, q.scheme, q.bc, var, -L, c->stream_general);
                 c->z_chunk = (c->nz + c->nteams - 1) / c->nteams;
             }
             // whole RK4 steps: the variants move different bytes per stage

@@ -135,19 +135,22 @@ hipError_t launch_rhs_fast(const RhsArgs &a, int z_chunk, int nteams, hipStream_
 int fast_resident_teams(int VP, int scheme, int bc);
 
 // Barrier-free FAST kernel (variant 3, stage_stream.hip): every wavefront
-// marches its own 32 bins in the upwind direction and takes the per-z moments
-// of its input from a_mom_in, written by the launch that produced the input.
-// Needs nvz a multiple of 64, periodic boundaries and bins of one sign of v
-// in every group of 32 (stream_supported).
+// marches its own 32 bins and takes the per-z moments of its input from
+// a.mom_in, written by the launch that produced the input.  The uniform form
+// (general = false) needs nvz a multiple of 64, periodic boundaries and bins
+// of one sign of v in every group of 32 (stream_supported) and marches each
+// group in its upwind direction; the general form takes any nvz (rows of VP,
+// a multiple of 64), groups that straddle v = 0 and either boundary.
 int stream_supported(int nvz, int VP, int bc, const double *vz_host);
 hipError_t launch_stage_stream(const StageArgs &a, int stage, int z_chunk, int nteams,
-                               hipStream_t s);
-// partial moments of state y into mom (kMomSlot doubles per z and bin group)
-hipError_t launch_stream_moments(const double *y, const double *vz, double *mom, int nz, int VP,
-                                 hipStream_t s);
+                               bool general, hipStream_t s);
+// partial moments of state y into mom (kMomSlot doubles per z and bin group;
+// bins v >= nvz count zero)
+hipError_t launch_stream_moments(const double *y, const double *vz, double *mom, int nz, int nvz,
+                                 int VP, hipStream_t s);
 // waves of the barrier-free kernel resident on the current device at once
-int stream_resident_waves(int scheme);
-const char *stage_stream_kernel_name(int stage);
+int stream_resident_waves(int scheme, bool general);
+const char *stage_stream_kernel_name(int stage, bool general);
 
 // Two RK4 stages per launch (variant 4, stage_pair.hip): pair 0 runs stages
 // 1 and 2 (a1, a2 = their stage arguments), pair 1 stages 3 and 4; the

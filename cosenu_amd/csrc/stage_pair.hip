@@ -35,6 +35,16 @@
 // in neighbouring teams' halos while it runs), and the state buffers swap
 // roles after every step.
 //
+// Open boundary (nuosc.hpp:247-277): stage-input rows below 0 read row 0 and
+// rows above nz-1 extrapolate linearly, one after another, from the last two
+// rows, where the window loads them.  The first stage's output at such ghost
+// rows follows the same rule instead of being computed: below the grid, row
+// 0's output is copied into the window positions of rows -4..-1 when it is
+// produced; above it, each ghost row is 2 x the previous row - the one
+// before.  The flux ghosts of the two boundary cells (rflux[-1] = rflux[0]
+// for v > 0 at cell 0, lflux[nz] = 2 lflux[nz-1] - lflux[nz-2] for v < 0 at
+// cell nz-1) keep the face before last of each stage in registers.
+//
 // Moments: the first stage's input moments are reduced when a row enters its
 // window (as in the stage kernel); the second stage's input moments are
 // reduced from the first stage's output row as it is produced.  Each goes
@@ -63,8 +73,8 @@ constexpr int kYRing = 8 * 4 * 64;
 // A1: the first stage of the pair (1 or 3), A2: the second (2 or 4) of the
 // same run.  PAIR 0: stages 1+2, PAIR 1: stages 3+4.  UNI: no padding bins
 // and every wavefront's bins of one sign of v (the common case): no masks
-// and no per-lane stencil selection.
-template <int NW, int SCHEME, int PAIR, bool UNI>
+// and no per-lane stencil selection.  OPENB: the open boundary.
+template <int NW, int SCHEME, int PAIR, bool UNI, bool OPENB>
 __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &A2, int L, int j0,
                                           int jend, bool active, double (*ra)[NW][8],
                                           double (*rb)[NW][8], double *yring)
@@ -95,8 +105,12 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             w -= nz;
         return w;
     };
+    // OPENB: rows below 0 read row 0 (the boundary rule); rows above nz-1
+    // read row nz-1 as a placeholder, and the window replaces them by the
+    // extrapolation when they enter it (ghost rows of y and Q are not used)
     auto load4 = [&](const double *base, int m, double o[4]) {
-        const double *r = base + (size_t)zrow(m) * zs + loff;
+        const int row = OPENB ? min(max(m, 0), nz - 1) : zrow(m);
+        const double *r = base + (size_t)row * zs + loff;
 #pragma unroll
         for (int f = 0; f < 4; f++)
             o[f] = r[(size_t)f * VP];
@@ -230,6 +244,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         load4(A1.yin, ibeg + 5, pre);
     __builtin_amdgcn_s_waitcnt(0);
     double Fa[4] = {0.0, 0.0, 0.0, 0.0}, Fb[4] = {0.0, 0.0, 0.0, 0.0};
+    double Fa2[4] = {0.0, 0.0, 0.0, 0.0}, Fb2[4] = {0.0, 0.0, 0.0, 0.0}; // OPENB FV: faces before last
     double wprev[4] = {0.0, 0.0, 0.0, 0.0}; // the row before the window
     double yn[4] = {0.0, 0.0, 0.0, 0.0};    // stages 3+4: y of the next first-stage cell
 
@@ -245,7 +260,10 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         }
         const int jb = i - 4;
         const bool cell_a = i >= j0 - 4 && i < jend + 4;
-        const bool face_b = jb >= j0 - 1 && jb < jend;
+        // OPENB: ghost rows of the first stage's output follow the boundary rule
+        const bool ghost_a = OPENB && (i < 0 || i >= nz);
+        // OPENB FV: one more face below the range, for the top cell's flux ghost
+        const bool face_b = jb >= j0 - (OPENB ? 2 : 1) && jb < jend;
         const bool cell_b = jb >= j0 && jb < jend;
         const int ra_row = zrow(i), rb_row = zrow(jb);
         // pointwise operands: y at the first stage's cell for stage 3 (loaded
@@ -284,22 +302,43 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
 
         // first stage at cell i: its output becomes the newest row of wb
         double oa[4] = {0.0, 0.0, 0.0, 0.0};
-        if (cell_a) {
-            double K[4];
-            rhs(wa, F, Fa, mxa, ra_row, K);
+        if (cell_a && !ghost_a) {
+            double K[4], Fl[4], Fr[4];
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                Fl[f] = Fa[f];
+                Fr[f] = F[f];
+                if (OPENB && SCHEME == 0 && i == 0 && pos)
+                    Fl[f] = F[f]; // rflux[-1] = rflux[0]
+                if (OPENB && SCHEME == 0 && i == nz - 1 && !pos)
+                    Fr[f] = Fa[f] * 2 - Fa2[f]; // lflux[nz], extrapolated
+            }
+            rhs(wa, Fr, Fl, mxa, ra_row, K);
 #pragma unroll
             for (int f = 0; f < 4; f++)
                 oa[f] = (PAIR == 0 ? wa[f][3] : ya[f]) + A1.a * K[f];
             if (coll)
                 moments(rb, i, oa[0], oa[1], oa[2], oa[3]);
+        } else if (cell_a && OPENB && i >= nz) {
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                oa[f] = wb[f][7] * 2 - wb[f][6]; // rows i-1, i-2
         }
 #pragma unroll
         for (int f = 0; f < 4; f++) {
+            if (OPENB)
+                Fa2[f] = Fa[f];
             Fa[f] = F[f];
 #pragma unroll
             for (int k = 0; k < 7; k++)
                 wb[f][k] = wb[f][k + 1];
             wb[f][7] = oa[f];
+        }
+        if (OPENB && i == 0 && cell_a) {
+            // rows -4..-1 of the first stage's output copy row 0
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                wb[f][3] = wb[f][4] = wb[f][5] = wb[f][6] = oa[f];
         }
         // second stage at cell jb: window wb holds rows jb-3 .. jb+4
         if (face_b) {
@@ -307,8 +346,17 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             if (SCHEME == 0 && advect)
                 face(wb, G);
             if (cell_b) {
-                double K[4];
-                rhs(wb, G, Fb, mxb, rb_row, K);
+                double K[4], Gl[4], Gr[4];
+#pragma unroll
+                for (int f = 0; f < 4; f++) {
+                    Gl[f] = Fb[f];
+                    Gr[f] = G[f];
+                    if (OPENB && SCHEME == 0 && jb == 0 && pos)
+                        Gl[f] = G[f];
+                    if (OPENB && SCHEME == 0 && jb == nz - 1 && !pos)
+                        Gr[f] = Fb[f] * 2 - Fb2[f];
+                }
+                rhs(wb, Gr, Gl, mxb, rb_row, K);
                 if (store) {
                     double *o = A2.yout + (size_t)rb_row * zs + loff;
                     double *oq = A1.yout + (size_t)rb_row * zs + loff; // Q (stages 1+2)
@@ -331,16 +379,23 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
                 }
             }
 #pragma unroll
-            for (int f = 0; f < 4; f++)
+            for (int f = 0; f < 4; f++) {
+                if (OPENB)
+                    Fb2[f] = Fb[f];
                 Fb[f] = G[f];
+            }
         }
+        // OPENB: row i+5 above the grid extrapolates from rows i+4, i+3, one
+        // row after another as the reference fills its ghost cells
+        const bool extrap = OPENB && i + 5 >= nz;
 #pragma unroll
         for (int f = 0; f < 4; f++) {
+            const double in = extrap ? wa[f][7] * 2 - wa[f][6] : nxt[f];
             wprev[f] = wa[f][0];
 #pragma unroll
             for (int k = 0; k < 7; k++)
                 wa[f][k] = wa[f][k + 1];
-            wa[f][7] = nxt[f];
+            wa[f][7] = in;
         }
     };
     if (SCHEME == 1) {
@@ -359,7 +414,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     }
 }
 
-template <int NW, int SCHEME, int PAIR, bool UNI>
+template <int NW, int SCHEME, int PAIR, bool UNI, bool OPENB>
 __global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_pair_stage(StageArgs A1, StageArgs A2, int L, int nchunks, int nteams)
 {
@@ -374,12 +429,13 @@ void cosenu_pair_stage(StageArgs A1, StageArgs A2, int L, int nchunks, int nteam
         const bool active = g < nteams;
         const int j0 = active ? (int)(((long)g * A1.nz) / nteams) : 0;
         const int j1 = active ? (int)(((long)(g + 1) * A1.nz) / nteams) : 0;
-        pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, j0, j1, active, ring_a[team], ring_b[team], yr);
+        pair_body<NW, SCHEME, PAIR, UNI, OPENB>(A1, A2, L, j0, j1, active, ring_a[team],
+                                                ring_b[team], yr);
     } else {
         const bool active = g < nchunks;
         const int j0 = (active ? g : 0) * L;
-        pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, j0, j0 + L, active, ring_a[team], ring_b[team],
-                                         yr);
+        pair_body<NW, SCHEME, PAIR, UNI, OPENB>(A1, A2, L, j0, j0 + L, active, ring_a[team],
+                                                ring_b[team], yr);
     }
 }
 
@@ -407,8 +463,8 @@ void cosenu_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int 
         A2.yout = r.acc;
     // a run's last item may be shorter than L (its nz need not be a multiple)
     const int jend = min(it.y + L, r.nz);
-    pair_body<NW, SCHEME, PAIR, UNI>(A1, A2, L, it.y, jend, active, ring_a[team], ring_b[team],
-                                     yring + (PAIR == 1 ? team * NW * kYRing : 0));
+    pair_body<NW, SCHEME, PAIR, UNI, false>(A1, A2, L, it.y, jend, active, ring_a[team],
+                                            ring_b[team], yring + (PAIR == 1 ? team * NW * kYRing : 0));
 }
 
 template <int NW, int SCHEME>
@@ -445,15 +501,20 @@ hipError_t launch_nw(const StageArgs &a1, const StageArgs &a2, int pair, int L, 
     }
     dim3 grid((work + TPB - 1) / TPB), block(64 * NW * TPB);
     // uniform: no padding bins and each group of 32 bins of one sign of v
-    // (v0 = -v1 and nvz a multiple of 64 give that; `uniform` is the host's check)
-    if (uni && pair == 0)
-        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 0, true>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+    // (v0 = -v1 and nvz a multiple of 64 give that; `uniform` is the host's
+    // check); the open boundary takes the general form
+    if (a1.bc == 1 && pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 0, false, true>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+    else if (a1.bc == 1)
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 1, false, true>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+    else if (uni && pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 0, true, false>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
     else if (uni)
-        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 1, true>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 1, true, false>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
     else if (pair == 0)
-        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 0, false>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 0, false, false>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
     else
-        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 1, false>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 1, false, false>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
     return hipGetLastError();
 }
 
@@ -478,7 +539,7 @@ int resident_teams_nw()
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void *>(&cosenu_pair_stage<NW, SCHEME, 1, false>),
+            &nb, reinterpret_cast<const void *>(&cosenu_pair_stage<NW, SCHEME, 1, false, false>),
             64 * NW * TPB, 0) != hipSuccess)
         return 0;
     return nb * prop.multiProcessorCount * TPB;
@@ -486,7 +547,10 @@ int resident_teams_nw()
 
 } // namespace
 
-int pair_supported(int VP, int bc) { return (VP == 64 || VP == 128 || VP == 256) && bc == 0; }
+int pair_supported(int VP, int bc)
+{
+    return (VP == 64 || VP == 128 || VP == 256) && (bc == 0 || bc == 1);
+}
 
 hipError_t launch_stage_pair(const StageArgs &a1, const StageArgs &a2, int pair, int L,
                              int nteams, bool uniform, hipStream_t s)

@@ -17,7 +17,7 @@ from dataclasses import dataclass, replace
 
 import numpy as np
 
-from . import RunParams, FV, FD, VAC, COLL, FAST
+from . import RunParams, FV, FD, VAC, COLL, FAST, OPEN
 
 
 def _vz(p) -> np.ndarray:
@@ -135,7 +135,8 @@ class Workload:
 
 def workload(name: str, mode: int = FAST, scheme: int = FV) -> Workload:
     """The named benchmark workloads.  Physics: mu = 1, omega = 0,
-    theta = 1e-6 deg, pmo = 1, CFL = 0.4, periodic, v in [-1, 1]."""
+    theta = 1e-6 deg, pmo = 1, CFL = 0.4, v in [-1, 1]; periodic unless
+    the name ends in _open."""
     base = dict(CFL=0.4, flags=VAC | COLL, omega=0.0, theta=1e-6, pmo=1.0, mu=1.0,
                 mode=mode, scheme=scheme)
     if name == "c0":   # nz=1024 on [-600,600], nvz=64, 2000 steps
@@ -144,9 +145,15 @@ def workload(name: str, mode: int = FAST, scheme: int = FV) -> Workload:
     if name == "c1":   # nz=65536 on [-1500,1500], nvz=128, 20000 steps, FV and FD
         return Workload(name, RunParams(nz=65536, nvz=128, z0=-1500, z1=1500, **base), 20000,
                         0.6, 0.53, 0.92, gaussian_seed=True)
+    if name == "c1_open":  # c1 with open boundaries
+        return Workload(name, RunParams(nz=65536, nvz=128, z0=-1500, z1=1500, bc=OPEN, **base),
+                        20000, 0.6, 0.53, 0.92, gaussian_seed=True)
     if name == "wide":  # nz=16384 on [-375,375] (c1's dz), nvz=512, c1's physics
         return Workload(name, RunParams(nz=16384, nvz=512, z0=-375, z1=375, **base), 20000,
                         0.6, 0.53, 0.92, gaussian_seed=True)
+    if name == "wide_open":  # as wide with nvz=480 (rows padded to 512) and open boundaries
+        return Workload(name, RunParams(nz=16384, nvz=480, z0=-375, z1=375, bc=OPEN, **base),
+                        20000, 0.6, 0.53, 0.92, gaussian_seed=True)
     if name == "c4":   # nz=2^21 on [-20000,20000], nvz=256, 5000 steps
         return Workload(name, RunParams(nz=2 ** 21, nvz=256, z0=-20000, z1=20000, **base), 5000,
                         0.6, 0.5, 0.9)

@@ -84,14 +84,14 @@ typedef struct cosenu_info {
     int fast_variant;    /* FAST mode kernel: 5 = all four RK4 stages per
                             launch (default for FD with 3rd-order dissipation,
                             periodic, nvz <= 256), 4 = two RK4 stages per
-                            launch (default otherwise), 3 = barrier-free
-                            (default for small problems), 2 = one stage per
-                            launch, two lanes per bin (nu / nubar halves;
-                            the open boundary); 0 in STRICT.  nvz > 256: 3 only
-                            (periodic, nvz a multiple of 64 with every group
-                            of 32 bins on one side of v = 0; otherwise
-                            cosenu_create returns COSENU_ENOTSUP and such grids
-                            run in STRICT mode) */
+                            launch (default otherwise, either boundary),
+                            3 = barrier-free (default for small periodic
+                            problems), 2 = one stage per launch, two lanes
+                            per bin (nu / nubar halves; nz < 16); 0 in STRICT.
+                            nvz > 256: 3 only, in a general form where a
+                            group of 32 bins straddles v = 0, bins are padded
+                            (nvz not a multiple of 64) or the boundary is
+                            open */
     int range_teams;     /* FAST mode: > 0 when each of this many teams streams
                             one contiguous z range (z_chunk = its length) */
 } cosenu_info;

@@ -60,7 +60,6 @@ def test_library_loads_without_a_gpu():
     (dict(nz=64, nvz=0), "nvz"),
     (dict(nz=64, nvz=2048), "nvz"),
     (dict(nz=4, nvz=64), "nz"),
-    (dict(nz=64, nvz=512, bc=1), "FAST mode with nvz > 256 needs periodic"),
     (dict(nz=64, nvz=64, scheme=0, flags=16 | 2), "KO5"),
 ])
 def test_invalid_parameters_are_returned_before_any_device_call(kw, match):
@@ -70,6 +69,25 @@ def test_invalid_parameters_are_returned_before_any_device_call(kw, match):
     import cosenu_amd as C
     with pytest.raises(C.CosenuError, match=match):
         C.Simulation(C.RunParams(mode=C.FAST, **kw))
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+@pytest.mark.parametrize("nvz", [300, 480, 512, 1000])
+@pytest.mark.parametrize("bc", [0, 1])
+def test_fast_mode_takes_wide_grids_with_either_boundary(nvz, bc):
+    """Rows wider than 256 bins take the barrier-free kernel, in its general
+    form where bin groups straddle v = 0, bins are padded or the boundary is
+    open: validation accepts every such grid.  Without a GPU the call then
+    fails at the first device call, not at validation."""
+    import cosenu_amd as C
+    try:
+        sim = C.Simulation(C.RunParams(nz=64, nvz=nvz, bc=bc, mode=C.FAST))
+    except C.CosenuError as e:
+        assert "hipSetDevice" in str(e) or "hip" in str(e).lower(), str(e)
+        assert "FAST mode" not in str(e)
+    else:
+        assert sim.info()["fast_variant"] == 3
+        sim.close()
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")

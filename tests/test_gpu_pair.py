@@ -166,3 +166,63 @@ def test_pair_rk4_combination_has_no_drift(gpu):
     drift = abs(out.sum() - s.sum()) / abs(s.sum())
     assert drift < 1e-14
     assert _rel(out, s) < 1e-13
+
+
+# The open boundary (nuosc.hpp:247-277): stage-input rows below the grid read
+# row 0, rows above it extrapolate linearly; the first stage's output follows
+# the same rule at its ghost rows, and the flux ghosts of the two boundary
+# cells follow the reference's (rflux[-1] = rflux[0], lflux[nz] extrapolated).
+def _open_state(po, seed):
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=seed, amp=1e-3)
+    # a z-dependent background so that the boundary rules matter
+    return s * (1.0 + 0.05 * np.sin(np.linspace(0.0, 3.0, po.nz)))[None, None, :]
+
+
+@pytest.mark.parametrize("scheme,flags,nvz", [
+    (O.FV, O.VAC | O.COLL, 64), (O.FV, O.VAC | O.MAT | O.COLL, 128), (O.FV, O.VAC | O.COLL, 100),
+    (O.FD, O.VAC | O.COLL, 128), (O.FD, O.COLL | O.KO5, 256), (O.FD, O.VAC | O.COLL, 40)])
+def test_pair_open_boundary_matches_oracle(gpu, scheme, flags, nvz):
+    po, pc = _pair(scheme, 96, nvz, flags, bc=O.OPEN)
+    s = _open_state(po, 39)
+    ref = O.step(po, s, 13)
+    print(f"pair open scheme={scheme} flags={flags} nvz={nvz}", flush=True)
+    with C.Simulation(pc) as sim:
+        assert sim.info()["fast_variant"] == 4
+        sim.upload(s)
+        sim.step(13)
+        out = sim.download()
+    assert _rel(out, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("chunk", ["2", "1", "-1", "-2"])
+@pytest.mark.parametrize("scheme", [O.FV, O.FD])
+def test_pair_open_boundary_launch_shapes_identical(gpu, monkeypatch, scheme, chunk):
+    """Ranges of 1 and 2 rows put the boundary cells in teams whose first
+    stage covers only ghost rows on one side: bit for bit the results of
+    chunks of 48 rows."""
+    po, pc = _pair(scheme, 96, 64, bc=O.OPEN)
+    s = _open_state(po, 40)
+    outs = []
+    for ch in ("48", chunk):
+        monkeypatch.setenv("COSENU_ZCHUNK", ch)
+        print(f"pair open scheme={scheme} chunk={ch}", flush=True)
+        with C.Simulation(pc) as sim:
+            sim.upload(s)
+            sim.step(5)
+            outs.append(sim.download())
+    assert np.array_equal(outs[0], outs[1])
+    assert _rel(outs[0], O.step(po, s, 5)) <= 1e-10
+
+
+def test_pair_open_boundary_agrees_with_split_lane_kernel(gpu, monkeypatch):
+    po, pc = _pair(O.FV, 512, 128, bc=O.OPEN)
+    s = _open_state(po, 41)
+    outs = {}
+    for v in ("2", "4"):
+        monkeypatch.setenv("COSENU_FAST_VARIANT", v)
+        with C.Simulation(pc) as sim:
+            assert sim.info()["fast_variant"] == int(v)
+            sim.upload(s)
+            sim.step(41)
+            outs[v] = sim.download()
+    assert _rel(outs["4"], outs["2"]) <= 1e-12

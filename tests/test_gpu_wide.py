@@ -92,3 +92,84 @@ def test_wide_diagnostics_conserve_trace(gpu):
 def test_wide_autotune_keeps_the_barrier_free_kernel(gpu):
     variant, chunk, ms = C.autotune(C.RunParams(nz=256, nvz=512, mode=C.FAST))
     assert variant == 3 and ms > 0
+
+
+# The general form of the barrier-free kernel: bin groups of 32 that straddle
+# v = 0 (nvz = 300, 480: the group holding bin nvz/2), padding bins (300 in
+# rows of 320, 480 in 512, 1000 in 1024) and the open boundary (nuosc.hpp:247-277:
+# ghost rows copy row 0 below the grid and extrapolate linearly above it).
+GENERAL = [
+    (O.FV, O.VAC | O.COLL, O.PERIODIC, 300),
+    (O.FD, O.VAC | O.COLL, O.PERIODIC, 300),
+    (O.FV, O.VAC | O.COLL, O.OPEN, 512),
+    (O.FD, O.VAC | O.COLL, O.OPEN, 512),
+    (O.FV, O.VAC | O.MAT | O.COLL, O.OPEN, 448),
+    (O.FD, O.COLL | O.KO5, O.PERIODIC, 480),
+    (O.FD, O.VAC | O.MAT | O.COLL | O.KO5, O.OPEN, 480),
+    (O.FV, O.VAC | O.COLL, O.PERIODIC, 1000),
+]
+
+
+@pytest.mark.parametrize("scheme,flags,bc,nvz", GENERAL)
+def test_general_rk4_matches_oracle(gpu, scheme, flags, bc, nvz):
+    nz = 64 if nvz < 1000 else 32
+    po, pc = _pair(scheme, nz, nvz, flags, bc=bc)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=45, amp=1e-3)
+    ref = O.step(po, s, 10)
+    print(f"general rk4 nvz={nvz} scheme={scheme} flags={flags} bc={bc}", flush=True)
+    with C.Simulation(pc) as sim:
+        assert sim.info()["fast_variant"] == 3
+        assert "general" in sim.stage_kernel_name(0)
+        sim.upload(s)
+        sim.step(10)
+        out = sim.download()
+    print(f"  rel {_rel(out, ref):.3e}", flush=True)
+    assert np.isfinite(out).all()
+    assert _rel(out, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("scheme,bc,nvz", [(O.FV, O.PERIODIC, 300), (O.FV, O.OPEN, 480),
+                                           (O.FD, O.OPEN, 300)])
+def test_general_rhs_matches_oracle(gpu, scheme, bc, nvz):
+    po, pc = _pair(scheme, 32, nvz, bc=bc)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=46, amp=1e-3)
+    ref = O.rhs(po, s)
+    print(f"general rhs nvz={nvz} scheme={scheme} bc={bc}", flush=True)
+    with C.Simulation(pc) as sim:
+        sim.upload(s)
+        out = sim.rhs()
+    assert _rel(out, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("zchunk", ["8", "1", "-2"])
+def test_general_launch_shapes_agree(gpu, monkeypatch, zchunk):
+    """Chunks of 8 rows, of 1 row (every FV range starts with the faces
+    below it) and contiguous ranges over twice the resident wavefronts."""
+    po, pc = _pair(O.FV, 64, 480, bc=O.OPEN)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=47, amp=1e-3)
+    ref = O.step(po, s, 4)
+    monkeypatch.setenv("COSENU_ZCHUNK", zchunk)
+    print(f"general zchunk={zchunk}", flush=True)
+    with C.Simulation(pc) as sim:
+        sim.upload(s)
+        sim.step(4)
+        out = sim.download()
+    assert _rel(out, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("scheme", [O.FV, O.FD])
+def test_general_fast_agrees_with_strict(gpu, scheme):
+    """Open boundary, mixed groups and padding over 60 steps of the
+    flavour-unstable regime, against STRICT mode's per-cell loop."""
+    po, pc = _pair(scheme, 256, 480, bc=O.OPEN)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=48, amp=1e-3)
+    outs = {}
+    for mode in (C.STRICT, C.FAST):
+        pc.mode = mode
+        print(f"general vs strict scheme={scheme} mode={mode}", flush=True)
+        with C.Simulation(pc) as sim:
+            sim.upload(s)
+            sim.step(60)
+            outs[mode] = sim.download()
+    assert np.isfinite(outs[C.FAST]).all()
+    assert _rel(outs[C.FAST], outs[C.STRICT]) <= 1e-10

@@ -419,7 +419,7 @@ hipError_t launch_stage_fast(const StageArgs &a, int stage, int L, int nteams, h
     } else if (L < 1 || a.nz % L != 0) {
         return hipErrorInvalidValue;
     }
-    if (a.bc == 1) // open boundary: split-lane kernel only
+    if (a.bc == 1) // the open boundary's instances
         return a.scheme == 0 ? launch_scheme2<0, true>(a, stage, L, nteams, s)
                              : launch_scheme2<1, true>(a, stage, L, nteams, s);
     return a.scheme == 0 ? launch_scheme2<0, false>(a, stage, L, nteams, s)

@@ -35,6 +35,14 @@
 // The new state is written to a second buffer (neighbouring teams read y in
 // their halos while the launch runs); the state buffers swap after every
 // step.
+//
+// Open boundary (nuosc.hpp:247-277; OPENB): y rows below the grid read row 0
+// (the load clamps its row index), rows above it enter the window as 2 x the
+// previous row - the one before (the reference's ghost fill, row by row).
+// Each stage's output at its ghost rows follows the same rule instead of
+// being computed: its row-0 output is copied into the window slots of rows
+// -2 and -1 when it is produced, and above the grid a ghost row is
+// extrapolated from the two rows before it in the stage's window.
 #include "fast_math.hpp"
 
 #include <cstdio>
@@ -59,8 +67,9 @@ constexpr int kQRing = (kTRing + kYRing) * 4 * 64; // doubles per wavefront: [sl
 
 // UNI: no padding bins (nvz == VP).  MAT: the matter term may be on (its
 // potential is then loaded ahead of use, which costs registers that runs
-// without it need not hold).  ca[0..3]: dt/2, dt/2, dt, dt/6.
-template <int NW, bool UNI, bool MAT>
+// without it need not hold).  OPENB: the open boundary.  ca[0..3]: dt/2,
+// dt/2, dt, dt/6.
+template <int NW, bool UNI, bool MAT, bool OPENB>
 __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, const double ca1,
                                           const double ca2, const double ca3, double *yout, int L,
                                           int j0, int jend, bool active, double (*rings)[kRing][NW][8],
@@ -84,6 +93,8 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
     double *const qr = qring + wit * kQRing;
 
     auto zrow = [&](int m) -> int {
+        if (OPENB)
+            return min(max(m, 0), nz - 1);
         int w = m < 0 ? m + nz : m; // m >= -12 > -nz (nz >= 16)
         while (w >= nz)
             w -= nz;
@@ -238,9 +249,24 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         if (coll)
             moments(rings[0], i + 2, w0[0][top0], w0[1][top0], w0[2][top0], w0[3][top0]);
 
+        // OPENB: a stage's output at a cell above the grid extrapolates from
+        // the two rows before it in the stage's window (slots (T+4)%5 and
+        // (T+3)%5 before the new row is written); at cell 0 it is also
+        // copied into the slots of rows -2 and -1
+        constexpr int prev1 = (T + 4) % 5, prev2 = (T + 3) % 5;
+        auto ghost = [&](double (&w)[4][5], int c, const double o[4]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                w[f][old] = c >= nz ? w[f][prev1] * 2 - w[f][prev2] : o[f];
+            if (c == 0) {
+#pragma unroll
+                for (int f = 0; f < 4; f++)
+                    w[f][prev1] = w[f][prev2] = o[f];
+            }
+        };
         // stage 1 at c1: Y2 = y + (dt/2) K1
         double o[4] = {0.0, 0.0, 0.0, 0.0};
-        if (c1 >= j0 - 6 && c1 < jend + 6) {
+        if (c1 >= j0 - 6 && c1 < jend + 6 && !(OPENB && (c1 < 0 || c1 >= nz))) {
             double K[4];
             rhs(w0, Rot<T>{}, mx[0], hmv[0], K);
 #pragma unroll
@@ -249,12 +275,16 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             if (coll)
                 moments(rings[1], c1, o[0], o[1], o[2], o[3]);
         }
+        if (OPENB) {
+            ghost(w1, c1, o);
+        } else {
 #pragma unroll
-        for (int f = 0; f < 4; f++)
-            w1[f][old] = o[f];
+            for (int f = 0; f < 4; f++)
+                w1[f][old] = o[f];
+        }
         // stage 2 at c2: Y3; Q and y to the rings.  y of this cell is the y
         // window's oldest row (shifted out at the end of the iteration).
-        if (c2 >= j0 - 4 && c2 < jend + 4) {
+        if (c2 >= j0 - 4 && c2 < jend + 4 && !(OPENB && (c2 < 0 || c2 >= nz))) {
             double K[4];
             rhs(w1, Rot<T + 1>{}, mx[1], hmv[1], K);
 #pragma unroll
@@ -271,11 +301,15 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             for (int f = 0; f < 4; f++)
                 o[f] = 0.0;
         }
+        if (OPENB) {
+            ghost(w2, c2, o);
+        } else {
 #pragma unroll
-        for (int f = 0; f < 4; f++)
-            w2[f][old] = o[f];
+            for (int f = 0; f < 4; f++)
+                w2[f][old] = o[f];
+        }
         // stage 3 at c3: Y4 = y + dt K3; T = y + (Q + (Y4 - y))/3 replaces Q
-        if (c3 >= j0 - 2 && c3 < jend + 2) {
+        if (c3 >= j0 - 2 && c3 < jend + 2 && !(OPENB && (c3 < 0 || c3 >= nz))) {
             double K[4];
             rhs(w2, Rot<T + 1>{}, mx[2], hmv[2], K);
 #pragma unroll
@@ -292,9 +326,13 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             for (int f = 0; f < 4; f++)
                 o[f] = 0.0;
         }
+        if (OPENB) {
+            ghost(w3, c3, o);
+        } else {
 #pragma unroll
-        for (int f = 0; f < 4; f++)
-            w3[f][old] = o[f];
+            for (int f = 0; f < 4; f++)
+                w3[f][old] = o[f];
+        }
         // stage 4 at c4: y' = T + (dt/6) K4
         if (c4 >= j0 && c4 < jend) {
             double K[4];
@@ -306,10 +344,12 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                     out[(size_t)f * VP] = fma(ca3, K[f], tslot(c4, f));
             }
         }
-        // the y window advances
+        // the y window advances (OPENB: row i+3 above the grid extrapolates
+        // from rows i+2 and i+1)
+        const bool ex = OPENB && i + 3 >= nz;
 #pragma unroll
         for (int f = 0; f < 4; f++) {
-            w0[f][old] = nxt[f];
+            w0[f][old] = ex ? w0[f][top0] * 2 - w0[f][prev2] : nxt[f];
             nxt[f] = nn[f];
         }
     };
@@ -331,7 +371,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
     }
 }
 
-template <int NW, bool UNI, bool MAT>
+template <int NW, bool UNI, bool MAT, bool OPENB>
 __global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_quad_stage(StageArgs A, double ca0, double ca1, double ca2, double ca3, double *yout,
                        int L, int nchunks, int nteams)
@@ -346,12 +386,13 @@ void cosenu_quad_stage(StageArgs A, double ca0, double ca1, double ca2, double c
         const bool active = g < nteams;
         const int j0 = active ? (int)(((long)g * A.nz) / nteams) : 0;
         const int j1 = active ? (int)(((long)(g + 1) * A.nz) / nteams) : 0;
-        quad_body<NW, UNI, MAT>(A, ca0, ca1, ca2, ca3, yout, L, j0, j1, active, rings[team], qr);
+        quad_body<NW, UNI, MAT, OPENB>(A, ca0, ca1, ca2, ca3, yout, L, j0, j1, active, rings[team],
+                                       qr);
     } else {
         const bool active = g < nchunks;
         const int j0 = (active ? g : 0) * L;
-        quad_body<NW, UNI, MAT>(A, ca0, ca1, ca2, ca3, yout, L, j0, j0 + L, active, rings[team],
-                                qr);
+        quad_body<NW, UNI, MAT, OPENB>(A, ca0, ca1, ca2, ca3, yout, L, j0, j0 + L, active,
+                                       rings[team], qr);
     }
 }
 
@@ -373,8 +414,8 @@ void cosenu_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int 
     const BatchRun &r = runs[it.x];
     const StageArgs A = batch_stage_args(r, 1, true);
     const int jend = min(it.y + L, r.nz);
-    quad_body<NW, UNI, MAT>(A, r.a[0], r.a[1], r.a[2], r.a[3], r.acc, L, it.y, jend, active,
-                       rings[team], qring + team * NW * kQRing);
+    quad_body<NW, UNI, MAT, false>(A, r.a[0], r.a[1], r.a[2], r.a[3], r.acc, L, it.y, jend, active,
+                                   rings[team], qring + team * NW * kQRing);
 }
 
 template <int NW>
@@ -411,18 +452,24 @@ hipError_t launch_nw(const StageArgs &a, const double ca[4], double *yout, int L
     }
     dim3 grid((work + TPB - 1) / TPB), block(64 * NW * TPB);
     const bool mat = (a.flags & kFlagMat) != 0;
-    if (uni && mat)
-        hipLaunchKernelGGL((cosenu_quad_stage<NW, true, true>), grid, block, 0, s, a, ca[0], ca[1],
-                           ca[2], ca[3], yout, L, nchunks, nteams);
-    else if (uni)
-        hipLaunchKernelGGL((cosenu_quad_stage<NW, true, false>), grid, block, 0, s, a, ca[0], ca[1],
-                           ca[2], ca[3], yout, L, nchunks, nteams);
-    else if (mat)
-        hipLaunchKernelGGL((cosenu_quad_stage<NW, false, true>), grid, block, 0, s, a, ca[0], ca[1],
-                           ca[2], ca[3], yout, L, nchunks, nteams);
-    else
-        hipLaunchKernelGGL((cosenu_quad_stage<NW, false, false>), grid, block, 0, s, a, ca[0],
+    if (a.bc == 1 && mat) // the open boundary: the general bin layout
+        hipLaunchKernelGGL((cosenu_quad_stage<NW, false, true, true>), grid, block, 0, s, a, ca[0],
                            ca[1], ca[2], ca[3], yout, L, nchunks, nteams);
+    else if (a.bc == 1)
+        hipLaunchKernelGGL((cosenu_quad_stage<NW, false, false, true>), grid, block, 0, s, a, ca[0],
+                           ca[1], ca[2], ca[3], yout, L, nchunks, nteams);
+    else if (uni && mat)
+        hipLaunchKernelGGL((cosenu_quad_stage<NW, true, true, false>), grid, block, 0, s, a, ca[0],
+                           ca[1], ca[2], ca[3], yout, L, nchunks, nteams);
+    else if (uni)
+        hipLaunchKernelGGL((cosenu_quad_stage<NW, true, false, false>), grid, block, 0, s, a, ca[0],
+                           ca[1], ca[2], ca[3], yout, L, nchunks, nteams);
+    else if (mat)
+        hipLaunchKernelGGL((cosenu_quad_stage<NW, false, true, false>), grid, block, 0, s, a, ca[0],
+                           ca[1], ca[2], ca[3], yout, L, nchunks, nteams);
+    else
+        hipLaunchKernelGGL((cosenu_quad_stage<NW, false, false, false>), grid, block, 0, s, a,
+                           ca[0], ca[1], ca[2], ca[3], yout, L, nchunks, nteams);
     return hipGetLastError();
 }
 
@@ -435,7 +482,7 @@ int resident_teams_nw()
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void *>(&cosenu_quad_stage<NW, false, true>), 64 * NW * TPB, 0) !=
+            &nb, reinterpret_cast<const void *>(&cosenu_quad_stage<NW, false, true, false>), 64 * NW * TPB, 0) !=
         hipSuccess)
         return 0;
     return nb * prop.multiProcessorCount * TPB;
@@ -445,7 +492,8 @@ int resident_teams_nw()
 
 int quad_supported(int VP, int bc, int scheme, int flags)
 {
-    return (VP == 64 || VP == 128 || VP == 256) && bc == 0 && scheme == 1 && !(flags & kFlagKO5);
+    return (VP == 64 || VP == 128 || VP == 256) && (bc == 0 || bc == 1) && scheme == 1 &&
+           !(flags & kFlagKO5);
 }
 
 hipError_t launch_stage_quad(const StageArgs &a, const double ca[4], double *yout, int L,

@@ -83,7 +83,7 @@ typedef struct cosenu_info {
     int z_chunk;         /* FAST mode: z cells per streaming work item */
     int fast_variant;    /* FAST mode kernel: 5 = all four RK4 stages per
                             launch (default for FD with 3rd-order dissipation,
-                            periodic, nvz <= 256), 4 = two RK4 stages per
+                            nvz <= 256, either boundary), 4 = two RK4 stages per
                             launch (default otherwise, either boundary),
                             3 = barrier-free (default for small periodic
                             problems), 2 = one stage per launch, two lanes

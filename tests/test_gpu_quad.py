@@ -7,8 +7,8 @@ every step.
 The stage arithmetic is that of the two-stage kernel (variant 4), so the
 results agree with it normwise (only the compiler's contraction of the final
 combination may differ), and with the reference arithmetic (the oracle)
-within the FAST bounds.  FV, 5th-order dissipation and open boundaries fall
-back to variant 4 / 2.
+within the FAST bounds.  FV and 5th-order dissipation fall back to
+variant 4.
 """
 import numpy as np
 import pytest
@@ -262,3 +262,72 @@ def test_quad_batch_bit_identical_to_solo_runs(gpu, monkeypatch, chunk):
     po = O.RunParams(nz=p.nz, nvz=p.nvz, z0=p.z0, z1=p.z1, scheme=O.FD, flags=p.flags,
                      omega=p.omega, theta=p.theta, mu=p.mu)
     assert _rel(outs[1], O.step(po, s0, 21)) <= 1e-10
+
+
+# The open boundary (nuosc.hpp:247-277): y rows below the grid read row 0,
+# rows above it extrapolate linearly; each stage's output follows the same
+# rule at its ghost rows.
+def _open_state(po, seed):
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=seed, amp=1e-3)
+    # a z-dependent background so that the boundary rules matter
+    return s * (1.0 + 0.05 * np.sin(np.linspace(0.0, 3.0, po.nz)))[None, None, :]
+
+
+@pytest.mark.parametrize("nvz", [40, 64, 128, 256])
+@pytest.mark.parametrize("flags", [O.VAC | O.COLL, O.VAC | O.MAT | O.COLL])
+def test_quad_open_boundary_matches_oracle(gpu, flags, nvz):
+    po, pc = _quad(96, nvz, flags, bc=O.OPEN)
+    s = _open_state(po, 51)
+    ref = O.step(po, s, 13)
+    print(f"quad open flags={flags} nvz={nvz}", flush=True)
+    with C.Simulation(pc) as sim:
+        assert sim.info()["fast_variant"] == 5
+        sim.upload(s)
+        sim.step(13)
+        out = sim.download()
+    assert _rel(out, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("nz", [16, 17, 23])
+def test_quad_open_boundary_small_grids(gpu, nz):
+    """Grids where every team's halo reaches both boundaries."""
+    po, pc = _quad(nz, 64, O.VAC | O.MAT | O.COLL, bc=O.OPEN)
+    s = _open_state(po, 52)
+    ref = O.step(po, s, 7)
+    print(f"quad open nz={nz}", flush=True)
+    with C.Simulation(pc) as sim:
+        assert sim.info()["fast_variant"] == 5
+        sim.upload(s)
+        sim.step(7)
+        out = sim.download()
+    assert _rel(out, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("chunk", ["2", "1", "-1", "-2"])
+def test_quad_open_boundary_launch_shapes_identical(gpu, monkeypatch, chunk):
+    po, pc = _quad(96, 64, bc=O.OPEN)
+    s = _open_state(po, 53)
+    outs = []
+    for ch in ("48", chunk):
+        monkeypatch.setenv("COSENU_ZCHUNK", ch)
+        print(f"quad open chunk={ch}", flush=True)
+        with C.Simulation(pc) as sim:
+            sim.upload(s)
+            sim.step(5)
+            outs.append(sim.download())
+    assert np.array_equal(outs[0], outs[1])
+    assert _rel(outs[0], O.step(po, s, 5)) <= 1e-10
+
+
+def test_quad_open_boundary_agrees_with_two_stage_kernel(gpu, monkeypatch):
+    po, pc = _quad(512, 128, O.VAC | O.MAT | O.COLL, bc=O.OPEN)
+    s = _open_state(po, 54)
+    outs = {}
+    for v in ("4", "5"):
+        monkeypatch.setenv("COSENU_FAST_VARIANT", v)
+        with C.Simulation(pc) as sim:
+            assert sim.info()["fast_variant"] == int(v)
+            sim.upload(s)
+            sim.step(41)
+            outs[v] = sim.download()
+    assert _rel(outs["5"], outs["4"]) <= 1e-12

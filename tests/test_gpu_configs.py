@@ -257,3 +257,34 @@ def test_surrogate_dataset_grid(gpu):
     solo = (acc / n).astype(np.float32)
     _log(f"surrogate dataset grid: solo run 0 equal to the batch: {np.array_equal(solo, Y[0])}")
     assert np.array_equal(solo, Y[0])
+
+
+def test_surrogate_screens_fresh_runs_at_dataset_grid(gpu):
+    """The committed ensemble (cosenu_amd/data/surrogate_ensemble.npz) screens
+    8 configurations it never saw, at the dataset's grid and schedule: the
+    accepted runs take the network's P_ee(v), the rejected ones the solver's;
+    the screened profiles are compared with the solver's on every run."""
+    import os
+    from cosenu_amd import surrogate as S
+    path = os.path.join(os.path.dirname(C.__file__), "data", "surrogate_ensemble.npz")
+    sur = S.Surrogate.load(path)
+    setup = S.SolverSetup(nz=8192, steps=8000)
+    elns = S.sample_elns(8, seed=4242)
+    X = np.stack([S.features(e) for e in elns])
+    pred, accept = sur.screen(X, gate="mean")
+    _log(f"surrogate screen: {int(accept.sum())} of 8 accepted (sweep-mean gate)")
+    Y, secs = S.solve(elns, setup)
+    _log(f"surrogate screen: solver on all 8 runs {secs:.1f} s")
+    assert np.isfinite(pred).all() and pred.min() > -0.05 and pred.max() < 1.05
+    screened = pred.copy()
+    screened[~accept] = Y[~accept]
+    err = S.run_errors(screened, Y)
+    err_net = S.run_errors(pred, Y)
+    _log(f"surrogate screen: per-run |dP_ee| network {np.round(err_net, 4).tolist()}, "
+         f"screened mean {err.mean():.4f}")
+    assert err[~accept].max(initial=0.0) == 0.0
+    # the screened runs' mean error at the level the gate is calibrated for
+    # (2% on a sweep's mean; 8 runs leave room for sampling spread)
+    assert err.mean() <= 0.04
+    for m in sur.members:
+        m.close()

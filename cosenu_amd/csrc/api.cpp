@@ -307,9 +307,12 @@ void tune_store(int VP, int scheme, int nz, TuneVal v)
 // as many teams (one wavefront per SIMD) finish sooner: below 8 rows per
 // resident team for FV, 3 for FD.  (nz = 1024, nvz = 64: 512 teams of 2
 // rows, 63 us per FV step against 90 us with 128 teams of 8 rows.)
-int range_teams(int nz, int VP, int scheme, int bc, int variant, int mult, bool general)
+// (stage4: the barrier-free kernel's stage-4 launch, which holds fewer
+// wavefronts per SIMD than stages 1-3 and gets ranges of its own)
+int range_teams(int nz, int VP, int scheme, int bc, int variant, int mult, bool general,
+                bool stage4 = false)
 {
-    long t = variant == 3   ? (long)stream_resident_waves(scheme, general) / (VP / 32)
+    long t = variant == 3   ? (long)stream_resident_waves(scheme, general, stage4) / (VP / 32)
              : variant == 4 ? (long)pair_resident_teams(VP, scheme)
              : variant == 5 ? (long)quad_resident_teams(VP)
                             : (long)fast_resident_teams(VP, scheme, bc);
@@ -353,6 +356,7 @@ struct cosenu_ctx {
     bool have_profile = false, have_state0 = false;
     int z_chunk = 0, variant = 2;
     int nteams = 0; // > 0: contiguous-range mode with this many teams
+    int nteams4 = 0; // variant 3 in range mode: the stage-4 launch's team count
     bool stream_ok = false; // the barrier-free kernel (variant 3) applies
     bool stream_general = false; // ... in its general form (nvz > 256 only)
     bool uniform = false;   // nvz == VP, every group of 32 bins of one sign of v
@@ -447,7 +451,9 @@ hipError_t launch_stage(const cosenu_ctx *c, int stage, hipStream_t s)
 {
     StageArgs a = stage_args(c, stage);
     if (c->p.mode == COSENU_FAST && c->variant == 3)
-        return launch_stage_stream(a, stage, c->z_chunk, c->nteams, c->stream_general, s);
+        return launch_stage_stream(a, stage, c->z_chunk,
+                                   stage == 4 && c->nteams4 > 0 ? c->nteams4 : c->nteams,
+                                   c->stream_general, s);
     if (c->p.mode == COSENU_FAST)
         return launch_stage_fast(a, stage, c->z_chunk, c->nteams, s);
     return launch_stage_strict(a, stage, s);
@@ -670,6 +676,9 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             c->nteams = range_teams(c->nz, c->VP, p->scheme, p->bc, c->variant, -chunk,
                                    c->stream_general);
             c->z_chunk = (c->nz + c->nteams - 1) / c->nteams;
+            if (c->variant == 3)
+                c->nteams4 = range_teams(c->nz, c->VP, p->scheme, p->bc, 3, -chunk,
+                                         c->stream_general, true);
         } else {
             if (chunk <= 0 || chunk > c->nz || c->nz % chunk)
                 chunk = default_chunk(c->nz, c->VP);
@@ -1105,12 +1114,16 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
                 rc = fail(COSENU_EHIP, "cosenu_autotune: launch failed");
                 break;
             }
+            c->nteams4 = 0;
             if (L > 0) {
                 c->z_chunk = L;
                 c->nteams = 0;
             } else {
                 c->nteams = range_teams(c->nz, c->VP, q.scheme, q.bc, var, -L, c->stream_general);
                 c->z_chunk = (c->nz + c->nteams - 1) / c->nteams;
+                if (var == 3)
+                    c->nteams4 = range_teams(c->nz, c->VP, q.scheme, q.bc, 3, -L,
+                                             c->stream_general, true);
             }
             // whole RK4 steps: the variants move different bytes per stage
             const int reps = 3;

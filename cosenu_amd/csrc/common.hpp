@@ -149,7 +149,7 @@ hipError_t launch_stage_stream(const StageArgs &a, int stage, int z_chunk, int n
 hipError_t launch_stream_moments(const double *y, const double *vz, double *mom, int nz, int nvz,
                                  int VP, hipStream_t s);
 // waves of the barrier-free kernel resident on the current device at once
-int stream_resident_waves(int scheme, bool general);
+int stream_resident_waves(int scheme, bool general, bool stage4 = false);
 const char *stage_stream_kernel_name(int stage, bool general);
 
 // Two RK4 stages per launch (variant 4, stage_pair.hip): pair 0 runs stages

@@ -337,7 +337,7 @@ void cosenu_stream_stage(StageArgs A, int L, int nwork, int nteams)
 // (phase + k) & 7; the row that enters next is loaded into registers ahead
 // of the compute and moved into the freed slot after it.
 template <int SCHEME, int STAGE, bool OPENB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGE == 4 ? 2 : 3)))
 void cosenu_stream_general(StageArgs A, int L, int nwork, int nteams)
 {
     const int lane = threadIdx.x & 63;
@@ -450,8 +450,17 @@ void cosenu_stream_general(StageArgs A, int L, int nwork, int nteams)
                       s4 = (p + 4) & 7, s5 = (p + 5) & 7, s6 = (p + 6) & 7, s7 = (p + 7) & 7;
         const bool more = m + 1 < n;
         const int r = j0 + m;
-        double nx[4]; // row r + 5, position 7 of the next iteration
-        load4(r + 5, nx);
+        // row r + 5, position 7 of the next iteration.  Loaded here, a full
+        // iteration ahead of use, except by FV stages 1-3, which load it once
+        // the faces are formed and the oldest row is dead: one row's
+        // registers fewer across the WENO reconstruction gives them 3
+        // wavefronts per SIMD instead of 2 (stage 4, HBM-bound at 2
+        // wavefronts either way, keeps the earlier load: 0.65 against 0.72
+        // ms at nz = 16384, nvz = 480)
+        constexpr bool late = SCHEME == 0 && STAGE != 4;
+        double nx[4];
+        if (!late)
+            load4(r + 5, nx);
         const bool cell = m >= 0;
         const size_t q = (size_t)(cell ? r : 0) * zs;
         double yv[4], y2v[4], y3v[4];
@@ -516,6 +525,8 @@ void cosenu_stream_general(StageArgs A, int L, int nwork, int nteams)
                 Fin[f] = Fo;
             }
         }
+        if (late)
+            load4(r + 5, nx);
         if (cell) {
             const double pe = w[0][s3], qe = w[1][s3], re = w[2][s3], se = w[3][s3];
             if (SCHEME == 1) {
@@ -671,16 +682,20 @@ hipError_t launch_scheme(const StageArgs &a, int stage, int L, int nteams, hipSt
     return launch_occ<SCHEME, 3>(a, stage, L, nteams, s);
 }
 
+// Resident wavefronts of the stage-2/3 launch or (stage4) of the stage-4
+// launch, which holds more registers and fewer wavefronts per SIMD.
 template <int SCHEME>
-int resident_waves(bool general)
+int resident_waves(bool general, bool stage4)
 {
     int nb = 0, dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return 0;
-    const void *k = general
-        ? reinterpret_cast<const void *>(&cosenu_stream_general<SCHEME, 2, false>)
-        : reinterpret_cast<const void *>(&cosenu_stream_stage<SCHEME, 2, 3>);
+    const void *k =
+        general ? (stage4 ? reinterpret_cast<const void *>(&cosenu_stream_general<SCHEME, 4, false>)
+                          : reinterpret_cast<const void *>(&cosenu_stream_general<SCHEME, 2, false>))
+                : (stage4 ? reinterpret_cast<const void *>(&cosenu_stream_stage<SCHEME, 4, 2>)
+                          : reinterpret_cast<const void *>(&cosenu_stream_stage<SCHEME, 2, 3>));
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 256, 0) != hipSuccess)
         return 0;
     return nb * prop.multiProcessorCount * 4;
@@ -739,9 +754,9 @@ hipError_t launch_stream_moments(const double *y, const double *vz, double *mom,
     return hipGetLastError();
 }
 
-int stream_resident_waves(int scheme, bool general)
+int stream_resident_waves(int scheme, bool general, bool stage4)
 {
-    return scheme == 0 ? resident_waves<0>(general) : resident_waves<1>(general);
+    return scheme == 0 ? resident_waves<0>(general, stage4) : resident_waves<1>(general, stage4);
 }
 
 const char *stage_stream_kernel_name(int stage, bool general)

@@ -108,6 +108,44 @@ def test_production_grid(gpu, scheme):
     assert not np.array_equal(f200, s0)
 
 
+@pytest.mark.parametrize("name,scheme,variant", [("c1_open", C.FV, 4), ("c1_open", C.FD, 5),
+                                                  ("wide_open", C.FV, 3), ("wide_open", C.FD, 3)])
+def test_open_boundary_full_size(gpu, name, scheme, variant):
+    """The open boundary at full size: nz = 65536 x 128 (two- and four-stage
+    kernels) and nz = 16384 x 480 (the general barrier-free kernel: a bin
+    group across v = 0 and padded rows).  FAST against the oracle (moment
+    form, 3 steps) and against STRICT (50 steps); STRICT's first step equal
+    to the oracle's reference arithmetic."""
+    tag = f"{name} {'FV' if scheme == C.FV else 'FD'}"
+    wl = workload(name, mode=C.FAST, scheme=scheme)
+    p = wl.params
+    po = _orc(p)
+    a, b, G1, bG1 = wl.factors()
+    s0 = a[:, :, None] * b[:, None, :]
+    _log(f"{tag}: nz={p.nz} nvz={p.nvz} open, FAST 3 + 47 steps")
+    with _start(p, a, b, G1, bG1) as sim:
+        assert sim.info()["fast_variant"] == variant
+        sim.step(3)
+        f3 = sim.download()
+        sim.step(47)
+        f50 = sim.download()
+    _log(f"{tag}: STRICT 1 + 49 steps")
+    with _start(dataclasses.replace(p, mode=C.STRICT), a, b, G1, bG1) as sim:
+        sim.step(1)
+        s1 = sim.download()
+        sim.step(49)
+        s50 = sim.download()
+    _log(f"{tag}: oracle, moment form 3 steps, reference arithmetic 1 step")
+    ref3 = O.step(po, s0, 3, moment_form=True)
+    ref1 = O.step(po, s0, 1)
+    r3, st50 = _rel(f3, ref3), _rel(f50, s50)
+    _log(f"{tag}: FAST-oracle after 3 steps {r3:.3e}; STRICT 1 step equal to the oracle: "
+         f"{np.array_equal(s1, ref1)}; FAST-STRICT after 50 steps {st50:.3e}")
+    assert r3 <= 1e-10 and st50 <= 1e-10
+    assert np.array_equal(s1, ref1)
+    assert np.isfinite(f50).all() and not np.array_equal(f50, s0)
+
+
 def _drift(d, d0, key):
     return abs(d[key] - d0[key]) / abs(d0[key])
 

@@ -14,8 +14,9 @@ The same JSON line carries
                 events on the library's stream, against 8 TB/s (algorithmic
                 bytes) plus its measured limiter (HBM or FP64 VALU);
   workloads     the other single-GPU workloads: the production grid with the
-                other scheme, and nz = 2^21 x nvz = 256 (181 GB resident) FV
-                and FD, each with its dominant kernel;
+                other scheme, nz = 2^21 x nvz = 256 (181 GB resident) FV and
+                FD, and the wide velocity grid nz = 16384 x nvz = 512 FV and
+                FD, each with its dominant kernel;
   sweep_speedup the surrogate-screened 256-run ELN sweep (nz = 8192, nvz = 64,
                 8000 steps) against the batched solver on the same runs;
   cpu_baseline  the reference's own CPU code on this host's cores.
@@ -57,8 +58,11 @@ BYTES_PER_CELL_STEP_DEF = sum(ALG_STAGE_BYTES)
 #   5 (FD: four stages per launch; stage outputs stay in registers, the
 #      increment sum in LDS): read y, write the new state (64+64), plus the
 #      halo rows each team reads again (12 rows per team range)
+#   6 (rows of 512 or 1024 bins, two stages per launch with a team over 2 or
+#      4 blocks): as 4; the per-row moment partials the blocks exchange
+#      (768 bytes per row and block, each way) are under 2 bytes per cell
 STAGE_BYTES = {2: (128, 192, 192, 320), 3: (132, 196, 196, 324), 4: (192, 256, 0, 0),
-               5: (128, 0, 0, 0)}
+               5: (128, 0, 0, 0), 6: (192, 256, 0, 0)}
 SURROGATE_FILE = os.path.join(ROOT, "cosenu_amd", "data", "surrogate_ensemble.npz")
 SURROGATE_RECORD = os.path.join(ROOT, "profiles", "surrogate_screened_sweep.json")
 
@@ -153,9 +157,10 @@ def _profile_record(kind, workload_name, scheme, launch):
 
 
 def measure(C, name, scheme, steps, warmup, timing_steps):
-    """Throughput of `steps` RK4 steps (graph replay, the graph captured before
-    the timed region) and per-launch durations from HIP events over
-    `timing_steps` more steps.  Returns (result dict, open Simulation)."""
+    """Per-launch durations from HIP events over `timing_steps` steps, then
+    the throughput of `steps` RK4 steps after `warmup` more (graph replay,
+    the graphs captured before anything is timed).  Returns (result dict,
+    open Simulation)."""
     from cosenu_amd.initial import workload
     wl = workload(name, mode=C.FAST, scheme=C.FV if scheme == "fv" else C.FD)
     p = wl.params
@@ -167,25 +172,28 @@ def measure(C, name, scheme, steps, warmup, timing_steps):
     sim.init_separable(a, b, C.STATE)
 
     # the step graphs (one per state buffer the step starts from) are captured
-    # before the warm-up, so the device runs the warm-up steps right up to the
-    # timed region instead of idling through a capture
+    # before anything is timed
     sim.prepare()
-    sim.step(warmup)
-    sim.sync()
-    t0 = time.perf_counter()
-    sim.step(steps)   # graph replay, as a caller steps
-    sim.sync()
-    el = time.perf_counter() - t0
-    # Per-launch durations: HIP events recorded around every launch on the
-    # library's stream.  (The events replace graph replay and add a gap per
-    # launch: at the 1024 x 64 size, where a step is ~30 us, they would
-    # lengthen it by a third, so the throughput above is timed without them.)
+    # Per-launch durations first: HIP events recorded around every launch on
+    # the library's stream, over `timing_steps` steps.  (The events replace
+    # graph replay and add a gap per launch: at the 1024 x 64 size, where a
+    # step is ~30 us, they would lengthen it by a third, so the throughput
+    # below is timed without them.)  Running this pass before the warm-up
+    # means the throughput window starts on the clock the chip holds under
+    # sustained load, as a long run sees it, not on the first ~25 ms after an
+    # idle GPU while the clock is still rising.
     sim.set_timing(True)
     sim.sync()
     sim.step(timing_steps)
     sim.sync()
     ms, cnt = sim.timing()
     sim.set_timing(False)
+    sim.step(warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(steps)   # graph replay, as a caller steps
+    sim.sync()
+    el = time.perf_counter() - t0
     d = sim.diagnostics()
 
     info = sim.info()
@@ -196,7 +204,7 @@ def measure(C, name, scheme, steps, warmup, timing_steps):
     # name: (avg ms per launch, algorithmic bytes/cell, moved bytes/cell, kernel, launches per step)
     if variant == 5:
         kernels = {"stages1-4": (avg[0], sum(ab), sb[0], sim.stage_kernel_name(0), 1)}
-    elif variant == 4:
+    elif variant in (4, 6):
         kernels = {"stages1+2": (avg[0], ab[0] + ab[1], sb[0], sim.stage_kernel_name(0), 1),
                    "stages3+4": (avg[1], ab[2] + ab[3], sb[1], sim.stage_kernel_name(1), 1)}
     else:
@@ -430,7 +438,8 @@ def main():
         # clock (20 steps after 2: the c1 FD step read 0.436 ms against
         # 0.395 ms of kernel time)
         side, side_warm = min(args.steps, 40), 16
-        others = [("c1", "fd" if args.scheme == "fv" else "fv"), ("c4", "fv"), ("c4", "fd")]
+        others = [("c1", "fd" if args.scheme == "fv" else "fv"), ("c4", "fv"), ("c4", "fd"),
+                  ("wide", "fv"), ("wide", "fd")]
         out["workloads"] = {}
         for name, scheme in others:
             if (name, scheme) == (args.workload, args.scheme):

@@ -265,7 +265,7 @@ void tune_load_locked()
         TuneVal v;
         // entries of kernels this build does not have are skipped
         if (ss >> k.VP >> k.scheme >> k.nz >> v.variant >> v.chunk >> v.ms && v.variant >= 2 &&
-            v.variant <= 5)
+            v.variant <= 6)
             g_tune[k] = v;
     }
 }
@@ -315,6 +315,7 @@ int range_teams(int nz, int VP, int scheme, int bc, int variant, int mult, bool 
     long t = variant == 3   ? (long)stream_resident_waves(scheme, general, stage4) / (VP / 32)
              : variant == 4 ? (long)pair_resident_teams(VP, scheme)
              : variant == 5 ? (long)quad_resident_teams(VP)
+             : variant == 6 ? (long)pair_wide_resident_blocks(scheme) / (VP / 256)
                             : (long)fast_resident_teams(VP, scheme, bc);
     t *= mult;
     if (variant == 4 && mult == 1 && t >= 2 && (nz + t - 1) / t < (scheme == 0 ? 8 : 3))
@@ -361,6 +362,11 @@ struct cosenu_ctx {
     bool stream_general = false; // ... in its general form (nvz > 256 only)
     bool uniform = false;   // nvz == VP, every group of 32 bins of one sign of v
     double *mom[4] = {nullptr, nullptr, nullptr, nullptr}; // its moments of y, A, B, acc
+    // variant 6: the device-memory moment rings of its multi-block teams
+    // (for xteams teams) and the 2 control words (ticket counter, error)
+    unsigned long long *xslots = nullptr;
+    unsigned long long *xctl = nullptr;
+    int xteams = 0;
     View view;
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -401,6 +407,10 @@ void release(cosenu_ctx *c)
     for (double *b : c->mom)
         if (b)
             (void)hipFree(b);
+    if (c->xslots)
+        (void)hipFree(c->xslots);
+    if (c->xctl)
+        (void)hipFree(c->xctl);
     for (hipEvent_t e : c->ev)
         (void)hipEventDestroy(e);
     for (hipGraphExec_t g : c->graph)
@@ -459,7 +469,10 @@ hipError_t launch_stage(const cosenu_ctx *c, int stage, hipStream_t s)
     return launch_stage_strict(a, stage, s);
 }
 
-bool two_stage_launches(const cosenu_ctx *c) { return c->p.mode == COSENU_FAST && c->variant == 4; }
+bool two_stage_launches(const cosenu_ctx *c)
+{
+    return c->p.mode == COSENU_FAST && (c->variant == 4 || c->variant == 6);
+}
 bool four_stage_launch(const cosenu_ctx *c) { return c->p.mode == COSENU_FAST && c->variant == 5; }
 // the kernels that write the new state to the spare buffer
 bool swaps_state(const cosenu_ctx *c) { return two_stage_launches(c) || four_stage_launch(c); }
@@ -471,7 +484,54 @@ hipError_t launch_pair(const cosenu_ctx *c, int pair, hipStream_t s)
     StageArgs a1 = stage_args(c, pair == 0 ? 1 : 3), a2 = stage_args(c, pair == 0 ? 2 : 4);
     if (pair == 1)
         a2.yout = c->acc;
+    if (c->variant == 6)
+        return c->nteams <= c->xteams ? launch_stage_pair_wide(a1, a2, pair, c->nteams, c->uniform,
+                                                               c->xslots, c->xctl, s)
+                                      : hipErrorInvalidValue;
     return launch_stage_pair(a1, a2, pair, c->z_chunk, c->nteams, c->uniform, s);
+}
+
+// Variant 6: moment rings for c->nteams teams (allocated or grown here, never
+// inside a graph capture) and zeroed control words.  The rings need no
+// clearing between launches of one team layout (their words are tagged with
+// launch and row); a new layout starts from zeroed rings and a zeroed ticket
+// counter (the kernels derive team, block and launch number from it).
+int wide_buffers(cosenu_ctx *c)
+{
+    if (c->xteams != c->nteams || !c->xslots) {
+        const size_t words = pair_wide_slot_words(c->VP, c->nteams);
+        if (c->nteams > c->xteams || !c->xslots) {
+            if (c->xslots)
+                (void)hipFree(c->xslots);
+            c->xslots = nullptr;
+            RC_TRY(dev_alloc(c->bytes, &c->xslots, words));
+        }
+        HIP_TRY(hipMemsetAsync(c->xslots, 0, words * sizeof(unsigned long long), c->stream));
+        c->xteams = std::max(c->xteams, c->nteams);
+    }
+    if (!c->xctl)
+        RC_TRY(dev_alloc(c->bytes, &c->xctl, 2));
+    HIP_TRY(hipMemsetAsync(c->xctl, 0, 2 * sizeof(unsigned long long), c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Variant 6: an error if a block of a launch since the last check gave up
+// waiting for its partners' moments (the launch's results are void).
+int wide_check(cosenu_ctx *c)
+{
+    if (!c->xctl)
+        return 0;
+    unsigned long long err = 0;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(&err, c->xctl + 1, sizeof(err), hipMemcpyDeviceToHost));
+    if (err) {
+        const unsigned long long zero = 0;
+        HIP_TRY(hipMemcpy(c->xctl + 1, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        return fail(COSENU_EHIP, "two-stage launch of a wide grid: a block waited more than 1 s "
+                                 "for the moments of its team's other blocks; the state is void");
+    }
+    return 0;
 }
 
 void swap_state(cosenu_ctx *c)
@@ -643,8 +703,10 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             chunk = c->nz % 2 ? 1 : 2;
         } else if (c->VP > 256) {
             // the team kernels hold a team of VP/32 wavefronts in one block
-            // of at most 8: wider grids take the barrier-free kernel
-            c->variant = 3;
+            // of at most 8: rows of 512 and 1024 bins take two stages per
+            // launch with a team over 2 or 4 blocks, other wide grids the
+            // barrier-free kernel
+            c->variant = pair_wide_supported(c->VP, p->bc) && c->nz >= 16 ? 6 : 3;
             chunk = -1;
         } else {
             // FD with 3rd-order dissipation: four stages per launch (nz =
@@ -658,14 +720,18 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             chunk = -1;
         }
         int ev = env_int("COSENU_FAST_VARIANT", 0), ez = env_int("COSENU_ZCHUNK", 0);
-        if (ev >= 2 && ev <= 5 && ev != c->variant) {
+        if (ev >= 2 && ev <= 6 && ev != c->variant) {
             c->variant = ev;
             chunk = -1; // that variant's default launch: contiguous ranges
         }
         if (ez != 0)
             chunk = ez;
-        if (c->VP > 256)
-            c->variant = 3; // the only kernel for nvz > 256 (stream_ok holds)
+        if (c->VP > 256 && !(c->variant == 6 && pair_wide_supported(c->VP, p->bc) && c->nz >= 16))
+            c->variant = 3; // the barrier-free kernel takes every wide grid (stream_ok holds)
+        if (c->variant == 6 && c->VP <= 256)
+            c->variant = 4;
+        if (c->variant == 6 && chunk > 0)
+            chunk = -1; // ranges only
         if (c->variant == 3 && !c->stream_ok)
             c->variant = 2; // shapes the barrier-free kernel does not take
         if (c->variant == 5 && (!quad_supported(c->VP, p->bc, p->scheme, p->flags) || c->nz < 16))
@@ -736,6 +802,14 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
         release(c);
         delete c;
         return fail(COSENU_EHIP, std::string("cosenu_create: ") + hipGetErrorString(e));
+    }
+    if (c->p.mode == COSENU_FAST && c->variant == 6) {
+        const int rcw = wide_buffers(c);
+        if (rcw) {
+            release(c);
+            delete c;
+            return rcw;
+        }
     }
     c->view.y = c->y;
     c->view.y0 = c->y0;
@@ -810,6 +884,7 @@ int cosenu_download_planes(cosenu_ctx *c, int which, double *const planes[8], pt
     if (!c || (which != COSENU_STATE && which != COSENU_STATE0))
         return fail(COSENU_EINVAL, "cosenu_download_planes: bad argument");
     HIP_TRY(hipSetDevice(c->p.device));
+    RC_TRY(wide_check(c));
     return view_download_planes(c->view, which_ptr(c->view, which), planes, stride, off, c->stream);
 }
 
@@ -830,6 +905,7 @@ int cosenu_download_interleaved(cosenu_ctx *c, int which, double *buf)
     if (!c)
         return fail(COSENU_EINVAL, "cosenu_download_interleaved: null context");
     HIP_TRY(hipSetDevice(c->p.device));
+    RC_TRY(wide_check(c));
     return view_download_interleaved(c->view, which, buf, c->stream);
 }
 
@@ -942,7 +1018,7 @@ int cosenu_sync(cosenu_ctx *c)
         return fail(COSENU_EINVAL, "cosenu_sync: null context");
     HIP_TRY(hipSetDevice(c->p.device));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    return 0;
+    return wide_check(c);
 }
 
 int cosenu_rhs(cosenu_ctx *c, double *const out[8], ptrdiff_t stride, int off)
@@ -953,12 +1029,20 @@ int cosenu_rhs(cosenu_ctx *c, double *const out[8], ptrdiff_t stride, int off)
         return fail(COSENU_EINVAL, "cosenu_rhs: bad layout");
     HIP_TRY(hipSetDevice(c->p.device));
     RhsArgs r = rhs_args(c, c->y, c->B);
-    if (c->p.mode == COSENU_FAST && c->variant == 3) {
+    // a single RHS of a wide grid: the barrier-free kernel (variant 6 has no
+    // one-stage form)
+    if (c->p.mode == COSENU_FAST && (c->variant == 3 || c->variant == 6)) {
         HIP_TRY(launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->nvz, c->VP, c->stream));
         StageArgs a = stage_args(c, 1);
         a.yout = c->B;
+        a.mom_in = c->mom[0];
         a.mom_out = nullptr;
-        HIP_TRY(launch_stage_stream(a, 0, c->z_chunk, c->nteams, c->stream_general, c->stream));
+        // variant 6's teams are not the barrier-free kernel's: its own ranges
+        const int nt = c->variant == 3 ? c->nteams
+                                       : range_teams(c->nz, c->VP, c->p.scheme, c->p.bc, 3, 1,
+                                                     c->stream_general);
+        const int zc = c->variant == 3 ? c->z_chunk : (c->nz + nt - 1) / nt;
+        HIP_TRY(launch_stage_stream(a, 0, zc, nt, c->stream_general, c->stream));
     } else if (c->p.mode == COSENU_FAST) { // variants 4, 5 evaluate single RHSs with variant 2
         HIP_TRY(launch_rhs_fast(r, c->z_chunk, c->nteams, c->stream));
     } else {
@@ -974,6 +1058,7 @@ int cosenu_diagnostics(cosenu_ctx *c, cosenu_diag *d)
     if (!c->have_profile || !c->have_state0)
         return fail(COSENU_ESTATE, "cosenu_diagnostics: set the profile and the t=0 state first");
     HIP_TRY(hipSetDevice(c->p.device));
+    RC_TRY(wide_check(c));
     return view_diagnostics(c->view, c->d.dz, c->d.dv, c->diag_part, c->npart, c->diag_out, d,
                             c->stream);
 }
@@ -985,6 +1070,7 @@ int cosenu_survival_profile(cosenu_ctx *c, double *pee, double *pbee)
     if (!c->have_state0)
         return fail(COSENU_ESTATE, "cosenu_survival_profile: set the t=0 state first");
     HIP_TRY(hipSetDevice(c->p.device));
+    RC_TRY(wide_check(c));
     return view_survival(c->view, c->diag_part, c->npart, c->diag_out, pee, pbee, c->stream);
 }
 
@@ -1071,6 +1157,8 @@ const char *cosenu_stage_kernel_name(cosenu_ctx *c, int stage)
         return stage_stream_kernel_name(stage, c->stream_general);
     if (four_stage_launch(c))
         return stage == 0 ? stage_quad_kernel_name() : "";
+    if (two_stage_launches(c) && c->variant == 6)
+        return stage < 2 ? stage_pair_wide_kernel_name(stage) : "";
     if (two_stage_launches(c))
         return stage < 2 ? stage_pair_kernel_name(stage) : "";
     return c->p.mode == COSENU_FAST ? stage_fast_kernel_name(c->VP, stage)
@@ -1098,15 +1186,16 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     if (!rc && (launch_fill_pattern(c->y, (long)c->nz * 8, c->nvz, c->VP, c->stream) != hipSuccess ||
                 launch_fill_pattern(c->prof, (long)c->nz * 2, c->nvz, c->VP, c->stream) != hipSuccess))
         rc = fail(COSENU_EHIP, "cosenu_autotune: fill failed");
-    // chunk lengths, then (variants 2-5) contiguous ranges with 1 or 2 times
+    // chunk lengths, then (variants 2-6) contiguous ranges with 1 or 2 times
     // the resident teams, encoded as chunk = -1, -2
     const int cands[] = {1, 2, 4, 8, 16, 32, 64, 128, 256, -1, -2};
-    for (int var = 2; var <= 5 && !rc; var++)
+    for (int var = 2; var <= 6 && !rc; var++)
         for (int L : cands) {
             if (rc || (L > 0 && (L > c->nz || c->nz % L)) ||
-                (var == 3 && !c->stream_ok) || (var != 3 && c->VP > 256) ||
+                (var == 3 && !c->stream_ok) || (var != 3 && var != 6 && c->VP > 256) ||
                 (var == 4 && (!pair_supported(c->VP, q.bc) || c->nz < 16)) ||
-                (var == 5 && (!quad_supported(c->VP, q.bc, q.scheme, q.flags) || c->nz < 16)))
+                (var == 5 && (!quad_supported(c->VP, q.bc, q.scheme, q.flags) || c->nz < 16)) ||
+                (var == 6 && (L > 0 || !pair_wide_supported(c->VP, q.bc) || c->nz < 16)))
                 continue;
             c->variant = var;
             if (var == 3 && launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->nvz,
@@ -1124,6 +1213,8 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
                 if (var == 3)
                     c->nteams4 = range_teams(c->nz, c->VP, q.scheme, q.bc, 3, -L,
                                              c->stream_general, true);
+                if (var == 6 && (rc = wide_buffers(c)) != 0)
+                    break;
             }
             // whole RK4 steps: the variants move different bytes per stage
             const int reps = 3;
@@ -1142,6 +1233,8 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
                 break;
             }
             ms /= reps;
+            if (var == 6 && (rc = wide_check(c)) != 0)
+                break;
             if (ms < best) {
                 best = ms;
                 bv = var;

@@ -169,6 +169,21 @@ int pair_resident_teams(int VP, int scheme);
 hipError_t launch_stage_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
                                    int scheme, int L, int pair, bool uniform, hipStream_t s);
 const char *stage_pair_kernel_name(int pair);
+// Two RK4 stages per launch for rows of 512 or 1024 bins (variant 6,
+// stage_pair.hip): a team of VP/256 blocks per z range, the per-row moments
+// summed over the blocks through `slots` (pair_wide_slot_words(VP, nteams)
+// words, zero before the first launch of a team layout) with `ctl` (2
+// words, zero before the first launch of a layout: the ticket counter and
+// the error word -- error != 0 after a block gave up waiting for a partner,
+// which voids the launch's results).  Range mode only: nteams contiguous z
+// ranges.
+int pair_wide_supported(int VP, int bc);
+size_t pair_wide_slot_words(int VP, int nteams);
+int pair_wide_resident_blocks(int scheme);
+hipError_t launch_stage_pair_wide(const StageArgs &a1, const StageArgs &a2, int pair, int nteams,
+                                  bool uniform, unsigned long long *slots,
+                                  unsigned long long *ctl, hipStream_t s);
+const char *stage_pair_wide_kernel_name(int pair);
 
 // FAST mode, FD with 3rd-order dissipation: all four RK4 stages in one
 // launch (variant 5, stage_quad.hip).  `a` = stage 1's arguments (yin = y);

@@ -70,20 +70,108 @@ static_assert(kRing - 5 >= kSync && kSync <= 4, "moment ring too shallow for the
 // [slot = row % 8][field][lane]
 constexpr int kYRing = 8 * 4 * 64;
 
+// Teams of NB > 1 blocks (rows wider than 256 bins, cosenu_pair_wide): each
+// block of 8 wavefronts owns 256 bins of the team's z range, and the moments
+// of a z row are summed over the team's blocks through device memory.  Every
+// wavefront publishes its 6 partials of a row (the values it writes to its
+// block's LDS ring) to a ring in device memory as well, and the partner
+// blocks read them 4 z steps later, when they need the row's totals.
+//
+// A value travels as two 64-bit words, each holding 32 bits of the double
+// and a 32-bit tag (the launch epoch and the z row): a word is written and
+// read with a single agent-scope atomic, so a reader sees either the old or
+// the new word, and a value is complete when both of its words carry the
+// tag it expects.  No fence orders the words, and slots need no clearing
+// between uses or launches (a stale word carries another row or epoch).
+//
+// Blocks form teams in the order they start (a ticket from a 64-bit atomic
+// counter that never resets: launch k of a layout takes tickets k * grid ..
+// (k + 1) * grid - 1), so a waiting block's partners have started already
+// or are the next blocks the dispatcher starts: no block waits on one that
+// cannot get a CU.  The epoch is the launch number k, so graph replays need
+// no memset node; the host zeroes the counter when the layout changes.  A
+// wait is bounded by the wall clock; a block that gives up sets the error
+// word (read by the host at its next synchronisation) and later waits end
+// at once, so a fault cannot hang the device.
+struct WideX {
+    unsigned long long *slots; // [2 rings][nteams][NB][kRing][64 elements][2 words]
+    unsigned long long *ctl;   // [0] ticket, [1] error
+    unsigned epoch;
+    int team, role, nteams;
+};
+constexpr long long kWaitTicks = 100000000; // 1 s of the 100 MHz wall clock
+
+struct XWords {
+    unsigned long long w0, w1;
+};
+
+// the partner words of the next rows of the two stages' moment rings
+template <int NB>
+struct XRows {
+    XWords a[NB - 1] = {}, b[NB - 1] = {};
+};
+template <>
+struct XRows<1> {
+};
+
+__device__ inline unsigned xtag(unsigned epoch, int row)
+{
+    return (epoch << 24) | ((unsigned)(row + 64) & 0xFFFFFFu);
+}
+
+__device__ inline void xput(unsigned long long *p, unsigned tag, double v)
+{
+    const unsigned long long t = (unsigned long long)tag << 32;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    __hip_atomic_store(p, t | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 1, t | (b & 0xFFFFFFFFull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline XWords xload(unsigned long long *p)
+{
+    return {__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+            __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+}
+
+// The value of a slot whose words were loaded into w: re-read until both
+// words carry `tag` (bounded wait, see above).
+__device__ inline double xvalue(XWords w, unsigned long long *p, unsigned tag,
+                                unsigned long long *ctl, bool &gave_up)
+{
+    auto ok = [&] { return (unsigned)(w.w0 >> 32) == tag && (unsigned)(w.w1 >> 32) == tag; };
+    if (!ok() && !gave_up) {
+        if (__hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull)
+            gave_up = true; // an earlier wait timed out: results are void anyway
+        const long long t0 = wall_clock64();
+        while (!gave_up && !ok()) {
+            __builtin_amdgcn_s_sleep(8);
+            w = xload(p);
+            if (!ok() && wall_clock64() - t0 > kWaitTicks) {
+                gave_up = true;
+                __hip_atomic_store(ctl + 1, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    return __longlong_as_double((long long)(((w.w0 & 0xFFFFFFFFull) << 32) | (w.w1 & 0xFFFFFFFFull)));
+}
+
 // A1: the first stage of the pair (1 or 3), A2: the second (2 or 4) of the
 // same run.  PAIR 0: stages 1+2, PAIR 1: stages 3+4.  UNI: no padding bins
 // and every wavefront's bins of one sign of v (the common case): no masks
 // and no per-lane stencil selection.  OPENB: the open boundary.
-template <int NW, int SCHEME, int PAIR, bool UNI, bool OPENB>
+// NB: blocks per team (1: the whole team in this block; > 1: this block holds
+// bins [X.role * NW * 32, (X.role + 1) * NW * 32) of the team's rows).
+template <int NW, int SCHEME, int PAIR, bool UNI, bool OPENB, int NB = 1>
 __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &A2, int L, int j0,
                                           int jend, bool active, double (*ra)[NW][8],
-                                          double (*rb)[NW][8], double *yring)
+                                          double (*rb)[NW][8], double *yring,
+                                          const WideX &X = WideX{})
 // ra, rb: [kRing][NW][8]; yring: the team's NW x kYRing doubles (stages 3+4)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wit = wave % NW;
     const int h = lane >> 5;
-    const int v = wit * 32 + (lane & 31);
+    const int v = (NB > 1 ? X.role * NW * 32 : 0) + wit * 32 + (lane & 31);
     const int VP = A1.VP, nz = A1.nz;
     const size_t zs = (size_t)8 * VP;
     const double sig = h ? -1.0 : 1.0;
@@ -127,16 +215,58 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     const double mudv = A1.mu * A1.dv;
     const double st = A1.st * A1.pmo, ct = A1.ct * A1.pmo;
 
+    // NB > 1: the device-memory slots of ring `rg` (0: first-stage input,
+    // 1: second-stage input), block b, z row `row`
+    // (32-bit word offsets: the rings of a launch hold at most a few million
+    // words; the team's and the second ring's parts are formed once)
+    const unsigned xteam = (unsigned)X.team * (NB * kRing * 128);
+    const unsigned xring = (unsigned)X.nteams * (NB * kRing * 128);
+    auto xslot = [&](int rg, int b, int row) -> unsigned long long * {
+        return X.slots + (xteam + (rg ? xring : 0u) + (unsigned)b * (kRing * 128) +
+                          (unsigned)(row & (kRing - 1)) * 128u);
+    };
+    bool gave_up = false;
     // partial moments of 4 field values of this lane into ring[row % kRing]
-    auto moments = [&](double (*ring)[NW][8], int row, double p, double q, double r, double s) {
+    // (and, NB > 1, into this block's device-memory ring rg)
+    auto moments = [&](double (*ring)[NW][8], int rg, int row, double p, double q, double r,
+                       double s) {
         const double S = live ? s : 0.0;
         const double R = live ? sig * r : 0.0;
         const double D = live ? sig * (p - q) : 0.0;
         const double m6[6] = {S, vi * S, R, vi * R, D, vi * D};
         const double t = wave_sum6(m6, lane);
         const int idx = (kMomentLane >> (4 * (lane >> 3))) & 15;
-        if ((lane & 7) == 0 && idx < 6)
+        if ((lane & 7) == 0 && idx < 6) {
             ring[row & (kRing - 1)][wit][idx] = t;
+            if (NB > 1)
+                xput(xslot(rg, X.role, row) + 2 * (wit * 8 + idx), xtag(X.epoch, row), t);
+        }
+    };
+    // NB > 1: this lane's element of a row's partials summed over the team's
+    // blocks, its own block's from LDS (`own`), partner k's (block role ^ k,
+    // k = 1 .. NB-1) from the words w[k-1] loaded earlier.  The sum is the
+    // tree (own + p1) + (p2 + p3): as IEEE addition commutes, every block of
+    // the team rounds it alike, (P0 + P1) + (P2 + P3) in block numbers.
+    const bool xlane = (lane & 7) < 6; // elements 6, 7 of a wavefront's row are unused
+    auto xload_row = [&](XWords *w, int rg, int row) {
+#pragma unroll
+        for (int k = 1; k < NB; k++)
+            w[k - 1] = xlane ? xload(xslot(rg, X.role ^ k, row) + 2 * lane) : XWords{0ull, 0ull};
+    };
+    auto xsum = [&](double own, const XWords *w, int rg, int row) {
+        double p[NB];
+        p[0] = own;
+#pragma unroll
+        for (int k = 1; k < NB; k++)
+            p[k] = xlane ? xvalue(w[k - 1], xslot(rg, X.role ^ k, row) + 2 * lane,
+                                  xtag(X.epoch, row), X.ctl, gave_up)
+                         : 0.0;
+        if constexpr (NB == 4)
+            return (p[0] + p[1]) + (p[2] + p[3]);
+        else if constexpr (NB == 2)
+            return p[0] + p[1];
+        else
+            return p[0];
     };
     // face value from a window, by the wavefront's upwind direction
     auto face = [&](const double w[4][8], double F[4]) {
@@ -225,9 +355,9 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         }
     }
     if (coll) {
-        moments(ra, ibeg + 1, wa[0][4], wa[1][4], wa[2][4], wa[3][4]);
-        moments(ra, ibeg + 2, wa[0][5], wa[1][5], wa[2][5], wa[3][5]);
-        moments(ra, ibeg + 3, wa[0][6], wa[1][6], wa[2][6], wa[3][6]);
+        moments(ra, 0, ibeg + 1, wa[0][4], wa[1][4], wa[2][4], wa[3][4]);
+        moments(ra, 0, ibeg + 2, wa[0][5], wa[1][5], wa[2][5], wa[3][5]);
+        moments(ra, 0, ibeg + 3, wa[0][6], wa[1][6], wa[2][6], wa[3][6]);
     }
     // The window's first rows have landed before the loop starts.  Without
     // this wait the compiler's wait-count analysis, which merges the loop
@@ -247,17 +377,32 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     double Fa2[4] = {0.0, 0.0, 0.0, 0.0}, Fb2[4] = {0.0, 0.0, 0.0, 0.0}; // OPENB FV: faces before last
     double wprev[4] = {0.0, 0.0, 0.0, 0.0}; // the row before the window
     double yn[4] = {0.0, 0.0, 0.0, 0.0};    // stages 3+4: y of the next first-stage cell
+    // NB > 1: the other blocks' partials of the rows the two stages take next
+    // (first stage: row i, published by the partners 4 iterations before it
+    // is needed; second stage: row jb, published when their first stage
+    // computed it), loaded one iteration ahead, right after the previous
+    // row's words were used, so that a load has most of an iteration to land
+    XRows<NB> xr; // (empty for NB = 1)
 
     auto iteration = [&](const int i) {
+        // NB > 1: is row r's total needed by the first (a) or second (b) stage
+        auto xneed_a = [&](int r) {
+            return coll && active && r >= j0 - 4 && r < jend + 4 && !(OPENB && (r < 0 || r >= nz));
+        };
+        auto xneed_b = [&](int r) { return coll && active && r >= j0 && r < jend; };
         double nxt[4];
-        if (PF) {
+        auto window_load = [&] {
+            if (PF) {
 #pragma unroll
-            for (int f = 0; f < 4; f++)
-                nxt[f] = pre[f];
-            load4(A1.yin, i + 6, pre);
-        } else {
-            load4(A1.yin, i + 5, nxt);
-        }
+                for (int f = 0; f < 4; f++)
+                    nxt[f] = pre[f];
+                load4(A1.yin, i + 6, pre);
+            } else {
+                load4(A1.yin, i + 5, nxt);
+            }
+        };
+        if (NB == 1)
+            window_load();
         const int jb = i - 4;
         const bool cell_a = i >= j0 - 4 && i < jend + 4;
         // OPENB: ghost rows of the first stage's output follow the boundary rule
@@ -265,6 +410,10 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         // OPENB FV: one more face below the range, for the top cell's flux ghost
         const bool face_b = jb >= j0 - (OPENB ? 2 : 1) && jb < jend;
         const bool cell_b = jb >= j0 && jb < jend;
+        // NB > 1: rows of partner words used by this iteration's stages
+        const bool xa = NB > 1 && xneed_a(i), xb = NB > 1 && xneed_b(jb);
+        if (NB > 1)
+            window_load();
         const int ra_row = zrow(i), rb_row = zrow(jb);
         // pointwise operands: y at the first stage's cell for stage 3 (loaded
         // one iteration ahead, into yn, and kept in the LDS ring for stage
@@ -288,7 +437,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
                 load4(A2.y2, jb, qb); // Q of the cell, written by stages 1+2
         }
         if (coll)
-            moments(ra, i + 4, wa[0][7], wa[1][7], wa[2][7], wa[3][7]);
+            moments(ra, 0, i + 4, wa[0][7], wa[1][7], wa[2][7], wa[3][7]);
         double F[4] = {0.0, 0.0, 0.0, 0.0};
         if (SCHEME == 0 && advect)
             face(wa, F);
@@ -297,8 +446,14 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             __syncthreads();
         // the two stages' input moments (rows stored 4 iterations ago, behind
         // a barrier), read before the first stage needs them
-        const double mxa = coll ? moment_partial<NW>(ra[i & (kRing - 1)], lane) : 0.0;
-        const double mxb = coll ? moment_partial<NW>(rb[jb & (kRing - 1)], lane) : 0.0;
+        double mxa = coll ? moment_partial<NW>(ra[i & (kRing - 1)], lane) : 0.0;
+        double mxb = coll ? moment_partial<NW>(rb[jb & (kRing - 1)], lane) : 0.0;
+        if constexpr (NB > 1) {
+            if (xa)
+                mxa = xsum(mxa, xr.a, 0, i);
+            if (xneed_a(i + 1))
+                xload_row(xr.a, 0, i + 1);
+        }
 
         // first stage at cell i: its output becomes the newest row of wb
         double oa[4] = {0.0, 0.0, 0.0, 0.0};
@@ -318,7 +473,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             for (int f = 0; f < 4; f++)
                 oa[f] = (PAIR == 0 ? wa[f][3] : ya[f]) + A1.a * K[f];
             if (coll)
-                moments(rb, i, oa[0], oa[1], oa[2], oa[3]);
+                moments(rb, 1, i, oa[0], oa[1], oa[2], oa[3]);
         } else if (cell_a && OPENB && i >= nz) {
 #pragma unroll
             for (int f = 0; f < 4; f++)
@@ -345,6 +500,12 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             double G[4] = {0.0, 0.0, 0.0, 0.0};
             if (SCHEME == 0 && advect)
                 face(wb, G);
+            if constexpr (NB > 1) {
+                if (xb)
+                    mxb = xsum(mxb, xr.b, 1, jb);
+                if (xneed_b(jb + 1))
+                    xload_row(xr.b, 1, jb + 1);
+            }
             if (cell_b) {
                 double K[4], Gl[4], Gr[4];
 #pragma unroll
@@ -384,6 +545,10 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
                     Fb2[f] = Fb[f];
                 Fb[f] = G[f];
             }
+        }
+        if constexpr (NB > 1) {
+            if (!face_b && xneed_b(jb + 1)) // (a range's first second-stage row has a face)
+                xload_row(xr.b, 1, jb + 1);
         }
         // OPENB: row i+5 above the grid extrapolates from rows i+4, i+3, one
         // row after another as the reference fills its ghost cells
@@ -465,6 +630,59 @@ void cosenu_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int 
     const int jend = min(it.y + L, r.nz);
     pair_body<NW, SCHEME, PAIR, UNI, false>(A1, A2, L, it.y, jend, active, ring_a[team],
                                             ring_b[team], yring + (PAIR == 1 ? team * NW * kYRing : 0));
+}
+
+// Rows of 512 or 1024 bins: a team of NB = VP/256 blocks of 8 wavefronts per
+// contiguous z range (range mode only: the grid is nteams * NB blocks, every
+// team the same trip count).  Teams form by ticket (see WideX).
+template <int SCHEME, int PAIR, bool UNI, bool OPENB, int NB>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
+void cosenu_pair_wide(StageArgs A1, StageArgs A2, int nteams, unsigned long long *slots,
+                      unsigned long long *ctl)
+{
+    constexpr int NW = 8;
+    __shared__ double ring_a[kRing][NW][8];
+    __shared__ double ring_b[kRing][NW][8];
+    __shared__ double yring[PAIR == 1 ? NW * kYRing : 1];
+    __shared__ unsigned long long ticket;
+    if (threadIdx.x == 0)
+        ticket = __hip_atomic_fetch_add(ctl, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const unsigned long long grid = gridDim.x;
+    const int t = (int)(ticket % grid);
+    WideX X;
+    X.slots = slots;
+    X.ctl = ctl;
+    X.epoch = (unsigned)(ticket / grid);
+    X.team = t / NB;
+    X.role = t % NB;
+    X.nteams = nteams;
+    const bool active = X.team < nteams;
+    const int L = (A1.nz + nteams - 1) / nteams;
+    const int j0 = active ? (int)(((long)X.team * A1.nz) / nteams) : 0;
+    const int j1 = active ? (int)(((long)(X.team + 1) * A1.nz) / nteams) : 0;
+    pair_body<NW, SCHEME, PAIR, UNI, OPENB, NB>(A1, A2, L, j0, j1, active, ring_a, ring_b, yring,
+                                                X);
+}
+
+template <int SCHEME, int NB>
+hipError_t launch_wide_nb(const StageArgs &a1, const StageArgs &a2, int pair, int nteams,
+                          bool uni, unsigned long long *slots, unsigned long long *ctl, hipStream_t s)
+{
+    dim3 grid(nteams * NB), block(512);
+    if (a1.bc == 1 && pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, false, true, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl);
+    else if (a1.bc == 1)
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, false, true, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl);
+    else if (uni && pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, true, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl);
+    else if (uni)
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, true, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl);
+    else if (pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, false, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl);
+    else
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, false, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl);
+    return hipGetLastError();
 }
 
 template <int NW, int SCHEME>
@@ -592,6 +810,53 @@ int pair_resident_teams(int VP, int scheme)
     case 256: return scheme == 0 ? resident_teams_nw<8, 0>() : resident_teams_nw<8, 1>();
     default: return 0;
     }
+}
+
+int pair_wide_supported(int VP, int bc)
+{
+    return (VP == 512 || VP == 1024) && (bc == 0 || bc == 1);
+}
+
+size_t pair_wide_slot_words(int VP, int nteams)
+{
+    return (size_t)2 * nteams * (VP / 256) * kRing * 64 * 2;
+}
+
+int pair_wide_resident_blocks(int scheme)
+{
+    int nb = 0, dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return 0;
+    const void *k = scheme == 0
+                        ? reinterpret_cast<const void *>(&cosenu_pair_wide<0, 1, false, false, 2>)
+                        : reinterpret_cast<const void *>(&cosenu_pair_wide<1, 1, false, false, 2>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 512, 0) != hipSuccess)
+        return 0;
+    return nb * prop.multiProcessorCount;
+}
+
+hipError_t launch_stage_pair_wide(const StageArgs &a1, const StageArgs &a2, int pair, int nteams,
+                                  bool uniform, unsigned long long *slots,
+                                  unsigned long long *ctl, hipStream_t s)
+{
+    if (!pair_wide_supported(a1.VP, a1.bc) || a1.nz < 16 || a1.nz > (1 << 23) ||
+        (pair != 0 && pair != 1) || nteams < 1 || nteams > a1.nz || !slots || !ctl)
+        return hipErrorInvalidValue;
+    if (uniform && a1.nvz != a1.VP)
+        return hipErrorInvalidValue;
+    if (a1.VP == 512)
+        return a1.scheme == 0 ? launch_wide_nb<0, 2>(a1, a2, pair, nteams, uniform, slots, ctl, s)
+                              : launch_wide_nb<1, 2>(a1, a2, pair, nteams, uniform, slots, ctl, s);
+    return a1.scheme == 0 ? launch_wide_nb<0, 4>(a1, a2, pair, nteams, uniform, slots, ctl, s)
+                          : launch_wide_nb<1, 4>(a1, a2, pair, nteams, uniform, slots, ctl, s);
+}
+
+const char *stage_pair_wide_kernel_name(int pair)
+{
+    static char buf[64];
+    snprintf(buf, sizeof(buf), "cosenu_pair_wide<*, %d>", pair);
+    return buf;
 }
 
 const char *stage_pair_kernel_name(int pair)

@@ -1,8 +1,11 @@
-"""FAST mode for velocity grids wider than 256 bins (nvz = 384 .. 1024).
+"""FAST mode for velocity grids wider than 256 bins (nvz = 300 .. 1024): the
+barrier-free kernel (variant 3).
 
-The team kernels keep the VP/32 wavefronts of a z range in one block of at
-most 8; wider grids take the barrier-free kernel (variant 3), in which each
-wavefront owns 32 bins and the per-z moments pass through memory as one
+The single-block team kernels keep the VP/32 wavefronts of a z range in one
+block of at most 8; the barrier-free kernel takes every wider grid (and is
+the default for rows other than 512 or 1024 bins, which take the multi-block
+two-stage kernel, variant 6, tested in test_gpu_pair_wide.py).  Each of its
+wavefronts owns 32 bins and the per-z moments pass through memory as one
 partial per (z row, group of 32 bins).  Beyond 8 groups a lane first adds
 the partials of groups g, g + 8, g + 16, ... and the fixed lane-move tree
 then adds the 8 results.  The reference runs any nvz (rhs_fv.hpp:188-233
@@ -17,6 +20,11 @@ import cosenu_amd as C
 from cosenu_amd.initial import seeded_ic
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def variant3(monkeypatch):
+    monkeypatch.setenv("COSENU_FAST_VARIANT", "3")
 
 
 def _pair(scheme, nz, nvz, flags=O.VAC | O.COLL, **kw):
@@ -89,9 +97,14 @@ def test_wide_diagnostics_conserve_trace(gpu):
     assert abs(d1["btrace"] - d0["btrace"]) <= 1e-12 * abs(d0["btrace"])
 
 
-def test_wide_autotune_keeps_the_barrier_free_kernel(gpu):
-    variant, chunk, ms = C.autotune(C.RunParams(nz=256, nvz=512, mode=C.FAST))
-    assert variant == 3 and ms > 0
+def test_wide_autotune_times_both_wide_kernels(gpu, monkeypatch):
+    """The autotuner times the barrier-free kernel and the multi-block
+    two-stage kernel on rows of 512 bins and records the faster."""
+    monkeypatch.delenv("COSENU_FAST_VARIANT")
+    variant, chunk, ms = C.autotune(C.RunParams(nz=1024, nvz=512, mode=C.FAST))
+    print(f"autotune nz=1024 nvz=512: variant {variant}, chunk {chunk}, {ms:.3f} ms", flush=True)
+    assert variant in (3, 6) and ms > 0
+    assert variant == 3 or chunk < 0  # the two-stage kernel runs contiguous ranges only
 
 
 # The general form of the barrier-free kernel: bin groups of 32 that straddle

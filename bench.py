@@ -63,6 +63,9 @@ BYTES_PER_CELL_STEP_DEF = sum(ALG_STAGE_BYTES)
 #      (768 bytes per row and block, each way) are under 2 bytes per cell
 STAGE_BYTES = {2: (128, 192, 192, 320), 3: (132, 196, 196, 324), 4: (192, 256, 0, 0),
                5: (128, 0, 0, 0), 6: (192, 256, 0, 0)}
+# untimed steps before the per-launch event pass (the clock settles over the
+# first ~25 ms of sustained work: ~20 steps of the headline workload)
+SETTLE_STEPS = 16
 SURROGATE_FILE = os.path.join(ROOT, "cosenu_amd", "data", "surrogate_ensemble.npz")
 SURROGATE_RECORD = os.path.join(ROOT, "profiles", "surrogate_screened_sweep.json")
 
@@ -157,10 +160,10 @@ def _profile_record(kind, workload_name, scheme, launch):
 
 
 def measure(C, name, scheme, steps, warmup, timing_steps):
-    """Per-launch durations from HIP events over `timing_steps` steps, then
-    the throughput of `steps` RK4 steps after `warmup` more (graph replay,
-    the graphs captured before anything is timed).  Returns (result dict,
-    open Simulation)."""
+    """Per-launch durations from HIP events over `timing_steps` steps (after
+    max(warmup, SETTLE_STEPS) untimed steps), then the throughput of `steps`
+    RK4 steps after `warmup` more (graph replay, the graphs captured before
+    anything is timed).  Returns (result dict, open Simulation)."""
     from cosenu_amd.initial import workload
     wl = workload(name, mode=C.FAST, scheme=C.FV if scheme == "fv" else C.FD)
     p = wl.params
@@ -174,14 +177,18 @@ def measure(C, name, scheme, steps, warmup, timing_steps):
     # the step graphs (one per state buffer the step starts from) are captured
     # before anything is timed
     sim.prepare()
-    # Per-launch durations first: HIP events recorded around every launch on
-    # the library's stream, over `timing_steps` steps.  (The events replace
+    # The chip's clock rises over the first ~25 ms of sustained work after
+    # the GPU was idle: neither measurement below starts before SETTLE_STEPS
+    # untimed steps (or the warm-up count, if larger) have run.
+    sim.step(max(warmup, SETTLE_STEPS))
+    sim.sync()
+    # Per-launch durations: HIP events recorded around every launch on the
+    # library's stream, over `timing_steps` steps.  (The events replace
     # graph replay and add a gap per launch: at the 1024 x 64 size, where a
     # step is ~30 us, they would lengthen it by a third, so the throughput
-    # below is timed without them.)  Running this pass before the warm-up
-    # means the throughput window starts on the clock the chip holds under
-    # sustained load, as a long run sees it, not on the first ~25 ms after an
-    # idle GPU while the clock is still rising.
+    # below is timed without them.)  The warm-up steps and the timed window
+    # follow, so the throughput is that of the clock the chip holds under
+    # sustained load, as a long run sees it.
     sim.set_timing(True)
     sim.sync()
     sim.step(timing_steps)
@@ -436,7 +443,7 @@ def main():
         # the other workloads: K steps (at most 40) after 16 warm-up steps,
         # one graph replay, so that their short timed region sees a settled
         # clock (20 steps after 2: the c1 FD step read 0.436 ms against
-        # 0.395 ms of kernel time)
+        # 0.395 ms of kernel time); each also settles before its event pass
         side, side_warm = min(args.steps, 40), 16
         others = [("c1", "fd" if args.scheme == "fv" else "fv"), ("c4", "fv"), ("c4", "fd"),
                   ("wide", "fv"), ("wide", "fd")]

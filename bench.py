@@ -17,6 +17,11 @@ The same JSON line carries
                 other scheme, nz = 2^21 x nvz = 256 (181 GB resident) FV and
                 FD, and the wide velocity grid nz = 16384 x nvz = 512 FV and
                 FD, each with its dominant kernel;
+  batched_sweep the 256-run batch of uneven runs (nz log-uniform in [4096,
+                32768], nvz in {64, 128}, random ELN parameters and phases)
+                in one graph-captured launch per step and stage group, FV and
+                FD: aggregate cell-updates/s and runs checked bit for bit
+                against the same runs advanced alone;
   sweep_speedup the surrogate-screened 256-run ELN sweep (nz = 8192, nvz = 64,
                 8000 steps) against the batched solver on the same runs;
   cpu_baseline  the reference's own CPU code on this host's cores.
@@ -456,6 +461,12 @@ def main():
             s2.close()
             r["bound"], r["hbm_use_of_copy"], r["valu_use_at_measured_clock"] = limiter(r, copy_gbs)
             out["workloads"][f"{name}_{scheme}"] = r
+        from cosenu_amd.sweep import run_sweep
+        out["batched_sweep"] = {}
+        for scheme in ("fv", "fd"):
+            log(f"batched sweep {scheme}: 256 runs, {side} steps after {side_warm}")
+            out["batched_sweep"][scheme] = run_sweep(256, steps=side, warmup=side_warm, check=2,
+                                                     scheme=scheme)
         out["sweep_speedup"] = sweep_speedup(C, window=160, log=log)
     if not args.no_cpu:
         log("cpu baseline")

@@ -44,19 +44,14 @@ def sweep_runs(n: int = 256, seed: int = 2026, scheme: int = FV):
     return runs, ics
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--runs", type=int, default=256)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=16)
-    ap.add_argument("--check", type=int, default=8)
-    ap.add_argument("--autotune", action="store_true")
-    ap.add_argument("--scheme", choices=["fv", "fd"], default="fv")
-    args = ap.parse_args(argv)
-
-    runs, ics = sweep_runs(args.runs, scheme=FD if args.scheme == "fd" else FV)
+def run_sweep(n_runs: int = 256, steps: int = 100, warmup: int = 16, check: int = 8,
+              scheme: str = "fv", autotune: bool = False) -> dict:
+    """Advance the sweep's runs in one batch: `warmup` untimed steps, then
+    `steps` timed ones; then `check` randomly chosen runs advanced alone for
+    the same number of steps and compared with the batch bit for bit."""
+    runs, ics = sweep_runs(n_runs, scheme=FD if scheme == "fd" else FV)
     tuned = None
-    if args.autotune:
+    if autotune:
         t0 = time.perf_counter()
         L, ms = batch_autotune(runs)
         tuned = {"z_chunk": L, "ms_per_step": ms, "seconds": time.perf_counter() - t0}
@@ -69,18 +64,18 @@ def main(argv=None):
         B.upload(r, s0, STATE)
     t_setup = time.perf_counter() - t0
     info = B.info()
-    B.step(args.warmup)
+    B.step(warmup)
     B.sync()
     t0 = time.perf_counter()
-    B.step(args.steps)
+    B.step(steps)
     B.sync()
     el = time.perf_counter() - t0
-    value = info["cells"] * args.steps / el
+    value = info["cells"] * steps / el
 
     # agreement with the same runs advanced alone
     rng = np.random.default_rng(1)
-    pick = sorted(rng.choice(len(runs), size=min(args.check, len(runs)), replace=False).tolist())
-    nsteps = args.warmup + args.steps
+    pick = sorted(rng.choice(len(runs), size=min(check, len(runs)), replace=False).tolist())
+    nsteps = warmup + steps
     same = 0
     for r in pick:
         p = runs[r]
@@ -92,16 +87,28 @@ def main(argv=None):
             solo = sim.download()
         same += int(np.array_equal(solo, B.download(r)))
     B.close()
-    out = {"metric": "batched sweep cell-updates/s", "value": value, "unit": "cell-updates/s",
-           "runs": len(runs), "scheme": args.scheme, "fast_variant": info["fast_variant"],
-           "cells": info["cells"], "work_items": info["work_items"],
-           "groups": info["ngroups"], "z_chunk": info["z_chunk"], "steps": args.steps,
-           "ms_per_step": 1e3 * el / args.steps, "setup_s": t_setup,
-           "device_GB": info["device_bytes"] / 1e9,
-           "checked_runs": pick, "bit_identical_to_solo": f"{same}/{len(pick)}",
-           "autotune": tuned}
+    return {"metric": "batched sweep cell-updates/s", "value": value, "unit": "cell-updates/s",
+            "runs": len(runs), "scheme": scheme, "fast_variant": info["fast_variant"],
+            "cells": info["cells"], "work_items": info["work_items"],
+            "groups": info["ngroups"], "z_chunk": info["z_chunk"], "steps": steps,
+            "warmup": warmup, "ms_per_step": 1e3 * el / steps, "setup_s": t_setup,
+            "device_GB": info["device_bytes"] / 1e9,
+            "checked_runs": pick, "bit_identical_to_solo": f"{same}/{len(pick)}",
+            "all_checked_identical": same == len(pick), "autotune": tuned}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--runs", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--check", type=int, default=8)
+    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--scheme", choices=["fv", "fd"], default="fv")
+    args = ap.parse_args(argv)
+    out = run_sweep(args.runs, args.steps, args.warmup, args.check, args.scheme, args.autotune)
     print(json.dumps(out), flush=True)
-    return 0 if same == len(pick) else 1
+    return 0 if out["all_checked_identical"] else 1
 
 
 if __name__ == "__main__":

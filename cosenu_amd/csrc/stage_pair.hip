@@ -96,10 +96,10 @@ constexpr int kYRing = 8 * 4 * 64;
 struct WideX {
     unsigned long long *slots; // [2 rings][nteams][NB][kRing][64 elements][2 words]
     unsigned long long *ctl;   // [0] ticket, [1] error
+    long long wait_ticks;      // the wait bound in wall-clock ticks (1 s)
     unsigned epoch;
     int team, role, nteams;
 };
-constexpr long long kWaitTicks = 100000000; // 1 s of the 100 MHz wall clock
 
 struct XWords {
     unsigned long long w0, w1;
@@ -136,7 +136,7 @@ __device__ inline XWords xload(unsigned long long *p)
 // The value of a slot whose words were loaded into w: re-read until both
 // words carry `tag` (bounded wait, see above).
 __device__ inline double xvalue(XWords w, unsigned long long *p, unsigned tag,
-                                unsigned long long *ctl, bool &gave_up)
+                                unsigned long long *ctl, long long wait_ticks, bool &gave_up)
 {
     auto ok = [&] { return (unsigned)(w.w0 >> 32) == tag && (unsigned)(w.w1 >> 32) == tag; };
     if (!ok() && !gave_up) {
@@ -146,7 +146,7 @@ __device__ inline double xvalue(XWords w, unsigned long long *p, unsigned tag,
         while (!gave_up && !ok()) {
             __builtin_amdgcn_s_sleep(8);
             w = xload(p);
-            if (!ok() && wall_clock64() - t0 > kWaitTicks) {
+            if (!ok() && wall_clock64() - t0 > wait_ticks) {
                 gave_up = true;
                 __hip_atomic_store(ctl + 1, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
@@ -259,7 +259,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
 #pragma unroll
         for (int k = 1; k < NB; k++)
             p[k] = xlane ? xvalue(w[k - 1], xslot(rg, X.role ^ k, row) + 2 * lane,
-                                  xtag(X.epoch, row), X.ctl, gave_up)
+                                  xtag(X.epoch, row), X.ctl, X.wait_ticks, gave_up)
                          : 0.0;
         if constexpr (NB == 4)
             return (p[0] + p[1]) + (p[2] + p[3]);
@@ -638,7 +638,7 @@ void cosenu_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int 
 template <int SCHEME, int PAIR, bool UNI, bool OPENB, int NB>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_pair_wide(StageArgs A1, StageArgs A2, int nteams, unsigned long long *slots,
-                      unsigned long long *ctl)
+                      unsigned long long *ctl, long long wait_ticks)
 {
     constexpr int NW = 8;
     __shared__ double ring_a[kRing][NW][8];
@@ -653,6 +653,7 @@ void cosenu_pair_wide(StageArgs A1, StageArgs A2, int nteams, unsigned long long
     WideX X;
     X.slots = slots;
     X.ctl = ctl;
+    X.wait_ticks = wait_ticks;
     X.epoch = (unsigned)(ticket / grid);
     X.team = t / NB;
     X.role = t % NB;
@@ -667,21 +668,22 @@ void cosenu_pair_wide(StageArgs A1, StageArgs A2, int nteams, unsigned long long
 
 template <int SCHEME, int NB>
 hipError_t launch_wide_nb(const StageArgs &a1, const StageArgs &a2, int pair, int nteams,
-                          bool uni, unsigned long long *slots, unsigned long long *ctl, hipStream_t s)
+                          bool uni, unsigned long long *slots, unsigned long long *ctl,
+                          long long ticks, hipStream_t s)
 {
     dim3 grid(nteams * NB), block(512);
     if (a1.bc == 1 && pair == 0)
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, false, true, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, false, true, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
     else if (a1.bc == 1)
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, false, true, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, false, true, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
     else if (uni && pair == 0)
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, true, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, true, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
     else if (uni)
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, true, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, true, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
     else if (pair == 0)
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, false, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, false, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
     else
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, false, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, false, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
     return hipGetLastError();
 }
 
@@ -845,11 +847,18 @@ hipError_t launch_stage_pair_wide(const StageArgs &a1, const StageArgs &a2, int 
         return hipErrorInvalidValue;
     if (uniform && a1.nvz != a1.VP)
         return hipErrorInvalidValue;
+    // the wait bound: 1 s of the device's constant-rate wall clock
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        khz = 100000;
+    const long long ticks = 1000LL * khz;
     if (a1.VP == 512)
-        return a1.scheme == 0 ? launch_wide_nb<0, 2>(a1, a2, pair, nteams, uniform, slots, ctl, s)
-                              : launch_wide_nb<1, 2>(a1, a2, pair, nteams, uniform, slots, ctl, s);
-    return a1.scheme == 0 ? launch_wide_nb<0, 4>(a1, a2, pair, nteams, uniform, slots, ctl, s)
-                          : launch_wide_nb<1, 4>(a1, a2, pair, nteams, uniform, slots, ctl, s);
+        return a1.scheme == 0
+                   ? launch_wide_nb<0, 2>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, s)
+                   : launch_wide_nb<1, 2>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, s);
+    return a1.scheme == 0 ? launch_wide_nb<0, 4>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, s)
+                          : launch_wide_nb<1, 4>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, s);
 }
 
 const char *stage_pair_wide_kernel_name(int pair)

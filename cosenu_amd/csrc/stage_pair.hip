@@ -64,6 +64,9 @@ namespace {
 using namespace fastk;
 
 constexpr int kRing = 16, kSync = 4;
+// the physics of the benchmark workloads: vacuum and collective terms with
+// advection, no matter term (FV: the dissipation order does not apply)
+constexpr int kStdPhys = kFlagVac | kFlagColl;
 static_assert(kRing - 5 >= kSync && kSync <= 4, "moment ring too shallow for the barrier spacing");
 
 // stages 3+4: y of the first stage's last 8 cells, per wavefront
@@ -161,13 +164,23 @@ __device__ inline double xvalue(XWords w, unsigned long long *p, unsigned tag,
 // and no per-lane stencil selection.  OPENB: the open boundary.
 // NB: blocks per team (1: the whole team in this block; > 1: this block holds
 // bins [X.role * NW * 32, (X.role + 1) * NW * 32) of the team's rows).
-template <int NW, int SCHEME, int PAIR, bool UNI, bool OPENB, int NB = 1>
+// PHYS: the physics switches (kFlag* bits) fixed at compile time, or -1 to
+// read them from the arguments (a fixed set drops the branches of the
+// right-hand side and the values they carry: FV at nz = 65536 x 128 runs
+// 3% faster so).
+template <int NW, int SCHEME, int PAIR, bool UNI, bool OPENB, int NB = 1, int PHYS = -1>
 __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &A2, int L, int j0,
                                           int jend, bool active, double (*ra)[NW][8],
                                           double (*rb)[NW][8], double *yring,
                                           const WideX &X = WideX{})
 // ra, rb: [kRing][NW][8]; yring: the team's NW x kYRing doubles (stages 3+4)
 {
+    // The right-hand side's terms and the RK4 combinations below write their
+    // fused multiply-adds out and nothing else is fused here (the WENO7 code
+    // keeps its own contraction): every instance of this body -- physics
+    // switches fixed or read at run time, single runs or batches -- rounds a
+    // cell alike, so batched runs stay bit-identical to single ones.
+#pragma clang fp contract(off)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wit = wave % NW;
     const int h = lane >> 5;
@@ -204,7 +217,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             o[f] = r[(size_t)f * VP];
     };
 
-    const int flags = A1.flags;
+    const int flags = PHYS >= 0 ? PHYS : A1.flags;
     const bool coll = (flags & kFlagColl) != 0;
     const bool vac = (flags & kFlagVac) != 0, mat = (flags & kFlagMat) != 0;
     const bool advect = !(flags & kFlagAdvecOff);
@@ -213,7 +226,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     const double fdfac = -vi / (12.0 * A1.dz);
     const double ko_eps = ko5 ? -A1.ko / A1.dz / 64.0 : -A1.ko / A1.dz / 16.0;
     const double mudv = A1.mu * A1.dv;
-    const double st = A1.st * A1.pmo, ct = A1.ct * A1.pmo;
+    const double st = A1.st * A1.pmo, ct = A1.ct * A1.pmo, st2 = 2.0 * st, ct2 = 2.0 * ct;
 
     // NB > 1: the device-memory slots of ring `rg` (0: first-stage input,
     // 1: second-stage input), block b, z row `row`
@@ -291,50 +304,41 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         const double p = w[0][3], qq = w[1][3], r = w[2][3], s = w[3][3];
 #pragma unroll
         for (int f = 0; f < 4; f++)
-            K[f] = 0.0;
-        if (advect) {
-            if (SCHEME == 0) {
-#pragma unroll
-                for (int f = 0; f < 4; f++)
-                    K[f] = fac * (F[f] - Fp[f]);
-            } else {
-#pragma unroll
-                for (int f = 0; f < 4; f++)
-                    K[f] = fdfac * ((w[f][1] - w[f][5]) - 8.0 * (w[f][2] - w[f][4]));
-            }
-        }
+            K[f] = !advect         ? 0.0
+                   : SCHEME == 0 ? fac * (F[f] - Fp[f])
+                                 : fdfac * fma(-8.0, w[f][2] - w[f][4], w[f][1] - w[f][5]);
         if (vac) {
-            K[0] += -2.0 * st * s;
-            K[1] += 2.0 * st * s;
-            K[2] += -2.0 * ct * s;
-            K[3] += 2.0 * ct * r + st * (p - qq);
+            K[0] = fma(-st2, s, K[0]);
+            K[1] = fma(st2, s, K[1]);
+            K[2] = fma(-ct2, s, K[2]);
+            K[3] = fma(ct2, r, fma(st, p - qq, K[3]));
         }
         if (mat) {
             const double hm = sig * A1.Hm[row];
-            K[2] += hm * s;
-            K[3] -= hm * r;
+            K[2] = fma(hm, s, K[2]);
+            K[3] = fma(-hm, r, K[3]);
         }
         if (coll) {
             double M[6];
             moment_totals<NW>(mx, M);
-            const double S = mudv * (M[0] - vi * M[1]);
-            const double R = mudv * (M[2] - vi * M[3]);
-            const double D = mudv * (M[4] - vi * M[5]);
-            const double I1 = 2.0 * (r * S - sig * s * R);
+            const double S = mudv * fma(-vi, M[1], M[0]);
+            const double R = mudv * fma(-vi, M[3], M[2]);
+            const double D = mudv * fma(-vi, M[5], M[4]);
+            const double ss = sig * s;
+            const double I1 = 2.0 * fma(r, S, -(ss * R));
             K[0] += I1;
             K[1] -= I1;
-            K[2] += (qq - p) * S + sig * s * D;
-            K[3] += sig * ((p - qq) * R - r * D);
+            K[2] = fma(qq - p, S, fma(ss, D, K[2]));
+            K[3] = fma(sig, fma(p - qq, R, -(r * D)), K[3]);
         }
         if (SCHEME == 1) {
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 const double *u = w[f];
-                if (ko5)
-                    K[f] += ko_eps * ((u[0] + u[6]) - 6.0 * (u[1] + u[5]) + 15.0 * (u[2] + u[4]) -
-                                      20.0 * u[3]);
-                else
-                    K[f] += ko_eps * ((u[1] + u[5]) - 4.0 * (u[2] + u[4]) + 6.0 * u[3]);
+                const double d = ko5 ? fma(-20.0, u[3], fma(15.0, u[2] + u[4],
+                                                            fma(-6.0, u[1] + u[5], u[0] + u[6])))
+                                     : fma(6.0, u[3], fma(-4.0, u[2] + u[4], u[1] + u[5]));
+                K[f] = fma(ko_eps, d, K[f]);
             }
         }
     };
@@ -471,7 +475,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             rhs(wa, Fr, Fl, mxa, ra_row, K);
 #pragma unroll
             for (int f = 0; f < 4; f++)
-                oa[f] = (PAIR == 0 ? wa[f][3] : ya[f]) + A1.a * K[f];
+                oa[f] = fma(A1.a, K[f], PAIR == 0 ? wa[f][3] : ya[f]);
             if (coll)
                 moments(rb, 1, i, oa[0], oa[1], oa[2], oa[3]);
         } else if (cell_a && OPENB && i >= nz) {
@@ -528,13 +532,13 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
                             // with y from the departed window row, Y2 = the
                             // window centre
                             const double y0 = wprev[f];
-                            const double y3 = y0 + A2.a * K[f];
+                            const double y3 = fma(A2.a, K[f], y0);
                             o[(size_t)f * VP] = y3;
-                            oq[(size_t)f * VP] = (wb[f][3] - y0) + 2.0 * (y3 - y0);
+                            oq[(size_t)f * VP] = fma(2.0, y3 - y0, wb[f][3] - y0);
                         } else { // y' = y + ((Q + (Y4 - y))/3 + (dt/6) K4)
                             const double y0 = yr[((jb & 7) * 4 + f) * 64 + lane];
                             const double d = qb[f] + (wb[f][3] - y0);
-                            o[(size_t)f * VP] = y0 + (d * (1.0 / 3.0) + A2.a * K[f]);
+                            o[(size_t)f * VP] = y0 + fma(A2.a, K[f], d * (1.0 / 3.0));
                         }
                     }
                 }
@@ -579,7 +583,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     }
 }
 
-template <int NW, int SCHEME, int PAIR, bool UNI, bool OPENB>
+template <int NW, int SCHEME, int PAIR, bool UNI, bool OPENB, int PHYS = -1>
 __global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_pair_stage(StageArgs A1, StageArgs A2, int L, int nchunks, int nteams)
 {
@@ -594,12 +598,12 @@ void cosenu_pair_stage(StageArgs A1, StageArgs A2, int L, int nchunks, int nteam
         const bool active = g < nteams;
         const int j0 = active ? (int)(((long)g * A1.nz) / nteams) : 0;
         const int j1 = active ? (int)(((long)(g + 1) * A1.nz) / nteams) : 0;
-        pair_body<NW, SCHEME, PAIR, UNI, OPENB>(A1, A2, L, j0, j1, active, ring_a[team],
+        pair_body<NW, SCHEME, PAIR, UNI, OPENB, 1, PHYS>(A1, A2, L, j0, j1, active, ring_a[team],
                                                 ring_b[team], yr);
     } else {
         const bool active = g < nchunks;
         const int j0 = (active ? g : 0) * L;
-        pair_body<NW, SCHEME, PAIR, UNI, OPENB>(A1, A2, L, j0, j0 + L, active, ring_a[team],
+        pair_body<NW, SCHEME, PAIR, UNI, OPENB, 1, PHYS>(A1, A2, L, j0, j0 + L, active, ring_a[team],
                                                 ring_b[team], yr);
     }
 }
@@ -635,7 +639,7 @@ void cosenu_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int 
 // Rows of 512 or 1024 bins: a team of NB = VP/256 blocks of 8 wavefronts per
 // contiguous z range (range mode only: the grid is nteams * NB blocks, every
 // team the same trip count).  Teams form by ticket (see WideX).
-template <int SCHEME, int PAIR, bool UNI, bool OPENB, int NB>
+template <int SCHEME, int PAIR, bool UNI, bool OPENB, int NB, int PHYS = -1>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_pair_wide(StageArgs A1, StageArgs A2, int nteams, unsigned long long *slots,
                       unsigned long long *ctl, long long wait_ticks)
@@ -662,8 +666,8 @@ void cosenu_pair_wide(StageArgs A1, StageArgs A2, int nteams, unsigned long long
     const int L = (A1.nz + nteams - 1) / nteams;
     const int j0 = active ? (int)(((long)X.team * A1.nz) / nteams) : 0;
     const int j1 = active ? (int)(((long)(X.team + 1) * A1.nz) / nteams) : 0;
-    pair_body<NW, SCHEME, PAIR, UNI, OPENB, NB>(A1, A2, L, j0, j1, active, ring_a, ring_b, yring,
-                                                X);
+    pair_body<NW, SCHEME, PAIR, UNI, OPENB, NB, PHYS>(A1, A2, L, j0, j1, active, ring_a, ring_b,
+                                                      yring, X);
 }
 
 template <int SCHEME, int NB>
@@ -676,6 +680,10 @@ hipError_t launch_wide_nb(const StageArgs &a1, const StageArgs &a2, int pair, in
         hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, false, true, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
     else if (a1.bc == 1)
         hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, false, true, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
+    else if (uni && SCHEME == 0 && a1.flags == kStdPhys && pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, true, false, NB, kStdPhys>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
+    else if (uni && SCHEME == 0 && a1.flags == kStdPhys)
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, true, false, NB, kStdPhys>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
     else if (uni && pair == 0)
         hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, true, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
     else if (uni)
@@ -727,6 +735,10 @@ hipError_t launch_nw(const StageArgs &a1, const StageArgs &a2, int pair, int L, 
         hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 0, false, true>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
     else if (a1.bc == 1)
         hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 1, false, true>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+    else if (uni && SCHEME == 0 && a1.flags == kStdPhys && pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 0, true, false, kStdPhys>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
+    else if (uni && SCHEME == 0 && a1.flags == kStdPhys)
+        hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 1, true, false, kStdPhys>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
     else if (uni && pair == 0)
         hipLaunchKernelGGL((cosenu_pair_stage<NW, SCHEME, 0, true, false>), grid, block, 0, s, a1, a2, L, nchunks, nteams);
     else if (uni)

@@ -367,6 +367,7 @@ struct cosenu_ctx {
     unsigned long long *xslots = nullptr;
     unsigned long long *xctl = nullptr;
     int xteams = 0;
+    bool wide_give_up_test = false; // COSENU_WIDE_GIVE_UP_TEST: force the error path
     View view;
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -485,9 +486,10 @@ hipError_t launch_pair(const cosenu_ctx *c, int pair, hipStream_t s)
     if (pair == 1)
         a2.yout = c->acc;
     if (c->variant == 6)
-        return c->nteams <= c->xteams ? launch_stage_pair_wide(a1, a2, pair, c->nteams, c->uniform,
-                                                               c->xslots, c->xctl, s)
-                                      : hipErrorInvalidValue;
+        return c->nteams <= c->xteams
+                   ? launch_stage_pair_wide(a1, a2, pair, c->nteams, c->uniform, c->xslots,
+                                            c->xctl, c->wide_give_up_test, s)
+                   : hipErrorInvalidValue;
     return launch_stage_pair(a1, a2, pair, c->z_chunk, c->nteams, c->uniform, s);
 }
 
@@ -804,6 +806,7 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
         return fail(COSENU_EHIP, std::string("cosenu_create: ") + hipGetErrorString(e));
     }
     if (c->p.mode == COSENU_FAST && c->variant == 6) {
+        c->wide_give_up_test = env_int("COSENU_WIDE_GIVE_UP_TEST", 0) != 0;
         const int rcw = wide_buffers(c);
         if (rcw) {
             release(c);

@@ -100,6 +100,7 @@ struct WideX {
     unsigned long long *slots; // [2 rings][nteams][NB][kRing][64 elements][2 words]
     unsigned long long *ctl;   // [0] ticket, [1] error
     long long wait_ticks;      // the wait bound in wall-clock ticks (1 s)
+    unsigned tag_flip;         // 0; a test of the give-up path expects tags no block writes
     unsigned epoch;
     int team, role, nteams;
 };
@@ -272,7 +273,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
 #pragma unroll
         for (int k = 1; k < NB; k++)
             p[k] = xlane ? xvalue(w[k - 1], xslot(rg, X.role ^ k, row) + 2 * lane,
-                                  xtag(X.epoch, row), X.ctl, X.wait_ticks, gave_up)
+                                  xtag(X.epoch, row) ^ X.tag_flip, X.ctl, X.wait_ticks, gave_up)
                          : 0.0;
         if constexpr (NB == 4)
             return (p[0] + p[1]) + (p[2] + p[3]);
@@ -642,7 +643,7 @@ void cosenu_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int 
 template <int SCHEME, int PAIR, bool UNI, bool OPENB, int NB, int PHYS = -1>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_pair_wide(StageArgs A1, StageArgs A2, int nteams, unsigned long long *slots,
-                      unsigned long long *ctl, long long wait_ticks)
+                      unsigned long long *ctl, long long wait_ticks, unsigned tag_flip)
 {
     constexpr int NW = 8;
     __shared__ double ring_a[kRing][NW][8];
@@ -658,6 +659,7 @@ void cosenu_pair_wide(StageArgs A1, StageArgs A2, int nteams, unsigned long long
     X.slots = slots;
     X.ctl = ctl;
     X.wait_ticks = wait_ticks;
+    X.tag_flip = tag_flip;
     X.epoch = (unsigned)(ticket / grid);
     X.team = t / NB;
     X.role = t % NB;
@@ -673,25 +675,25 @@ void cosenu_pair_wide(StageArgs A1, StageArgs A2, int nteams, unsigned long long
 template <int SCHEME, int NB>
 hipError_t launch_wide_nb(const StageArgs &a1, const StageArgs &a2, int pair, int nteams,
                           bool uni, unsigned long long *slots, unsigned long long *ctl,
-                          long long ticks, hipStream_t s)
+                          long long ticks, unsigned flip, hipStream_t s)
 {
     dim3 grid(nteams * NB), block(512);
     if (a1.bc == 1 && pair == 0)
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, false, true, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, false, true, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks, flip);
     else if (a1.bc == 1)
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, false, true, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, false, true, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks, flip);
     else if (uni && SCHEME == 0 && a1.flags == kStdPhys && pair == 0)
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, true, false, NB, kStdPhys>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, true, false, NB, kStdPhys>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks, flip);
     else if (uni && SCHEME == 0 && a1.flags == kStdPhys)
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, true, false, NB, kStdPhys>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, true, false, NB, kStdPhys>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks, flip);
     else if (uni && pair == 0)
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, true, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, true, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks, flip);
     else if (uni)
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, true, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, true, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks, flip);
     else if (pair == 0)
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, false, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 0, false, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks, flip);
     else
-        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, false, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks);
+        hipLaunchKernelGGL((cosenu_pair_wide<SCHEME, 1, false, false, NB>), grid, block, 0, s, a1, a2, nteams, slots, ctl, ticks, flip);
     return hipGetLastError();
 }
 
@@ -852,7 +854,7 @@ int pair_wide_resident_blocks(int scheme)
 
 hipError_t launch_stage_pair_wide(const StageArgs &a1, const StageArgs &a2, int pair, int nteams,
                                   bool uniform, unsigned long long *slots,
-                                  unsigned long long *ctl, hipStream_t s)
+                                  unsigned long long *ctl, bool give_up_test, hipStream_t s)
 {
     if (!pair_wide_supported(a1.VP, a1.bc) || a1.nz < 16 || a1.nz > (1 << 23) ||
         (pair != 0 && pair != 1) || nteams < 1 || nteams > a1.nz || !slots || !ctl)
@@ -864,13 +866,18 @@ hipError_t launch_stage_pair_wide(const StageArgs &a1, const StageArgs &a2, int 
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
         khz = 100000;
-    const long long ticks = 1000LL * khz;
+    // give_up_test: every wait looks for a tag no block writes and gives up
+    // after 10 us, so that the error path runs with every kernel still
+    // ending normally
+    const long long ticks = give_up_test ? khz / 100 : 1000LL * khz;
+    const unsigned flip = give_up_test ? 0x80000000u : 0u;
     if (a1.VP == 512)
         return a1.scheme == 0
-                   ? launch_wide_nb<0, 2>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, s)
-                   : launch_wide_nb<1, 2>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, s);
-    return a1.scheme == 0 ? launch_wide_nb<0, 4>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, s)
-                          : launch_wide_nb<1, 4>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, s);
+                   ? launch_wide_nb<0, 2>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, flip, s)
+                   : launch_wide_nb<1, 2>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, flip, s);
+    return a1.scheme == 0
+               ? launch_wide_nb<0, 4>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, flip, s)
+               : launch_wide_nb<1, 4>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, flip, s);
 }
 
 const char *stage_pair_wide_kernel_name(int pair)

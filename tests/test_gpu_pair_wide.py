@@ -170,3 +170,28 @@ def test_pair_wide_state_io_and_diagnostics(gpu):
     assert _rel(s41, O.step(po, a, 41, moment_form=True)) <= 1e-10
     assert _rel(r, O.rhs(po, b)) <= 1e-12
     assert _rel(s3, O.step(po, b, 3)) <= 1e-10
+
+
+def test_pair_wide_give_up_is_reported(gpu, monkeypatch):
+    """The bounded wait's error path, forced from the host: with
+    COSENU_WIDE_GIVE_UP_TEST every wait looks for a tag no block writes and
+    gives up after 10 us (every kernel still ends normally); the next
+    synchronisation reports the error.  A context made afterwards runs
+    normally."""
+    po, pc = _pair(O.FV, 64, 512)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=67, amp=1e-3)
+    monkeypatch.setenv("COSENU_WIDE_GIVE_UP_TEST", "1")
+    print("pair wide give-up test: forced", flush=True)
+    with C.Simulation(pc) as sim:
+        sim.upload(s)
+        sim.step(3)
+        with pytest.raises(C.CosenuError, match="waited more than"):
+            sim.sync()
+        sim.sync()  # the error is reported once; the word is cleared
+    monkeypatch.delenv("COSENU_WIDE_GIVE_UP_TEST")
+    print("pair wide give-up test: normal context", flush=True)
+    with C.Simulation(pc) as sim:
+        sim.upload(s)
+        sim.step(3)
+        out = sim.download()
+    assert _rel(out, O.step(po, s, 3)) <= 1e-10

@@ -1280,6 +1280,7 @@ struct Group {
     bool uniform = false; // every run: nvz == VP, groups of 32 bins of one sign of v
     bool quad = false;    // four stages per launch (FD, 3rd-order dissipation), else two
     bool any_mat = false; // some run has the matter term
+    bool std_phys = true; // every run: vacuum + collective, no matter term, KO3
 };
 
 } // namespace
@@ -1343,10 +1344,10 @@ int batch_enqueue_step(cosenu_batch *b, hipStream_t s)
                 if (g.quad) {
                     if (pair == 0)
                         HIP_TRY(launch_stage_quad_batch(runs, g.d_items, g.nitems, g.VP, g.L,
-                                                        g.uniform, g.any_mat, s));
+                                                        g.uniform, g.any_mat, g.std_phys, s));
                 } else {
                     HIP_TRY(launch_stage_pair_batch(runs, g.d_items, g.nitems, g.VP, g.scheme,
-                                                    g.L, pair, g.uniform, s));
+                                                    g.L, pair, g.uniform, g.std_phys, s));
                 }
             }
         b->parity ^= 1;
@@ -1498,6 +1499,7 @@ int cosenu_batch_create(const cosenu_params *runs, int nruns, cosenu_batch **out
         for (int r : g.runs) {
             g.quad = g.quad && quad_supported(g.VP, COSENU_PERIODIC, runs[r].scheme, runs[r].flags);
             g.any_mat = g.any_mat || (runs[r].flags & COSENU_MAT) != 0;
+            g.std_phys = g.std_phys && runs[r].flags == (COSENU_VAC | COSENU_COLL);
         }
     }
     int rc = 0;

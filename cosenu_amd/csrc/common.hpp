@@ -166,8 +166,12 @@ int pair_resident_teams(int VP, int scheme);
 // Batched two-stage launch over work items (run, first z cell) of runs with
 // the same VP and scheme; each run's .y is its current state and .acc the
 // buffer that receives the new one (pair 1).
+// std_phys: every run of the launch has the benchmark physics (vacuum +
+// collective, no matter term, 3rd-order dissipation): the instances with
+// those switches fixed at compile time
 hipError_t launch_stage_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
-                                   int scheme, int L, int pair, bool uniform, hipStream_t s);
+                                   int scheme, int L, int pair, bool uniform, bool std_phys,
+                                   hipStream_t s);
 const char *stage_pair_kernel_name(int pair);
 // Two RK4 stages per launch for rows of 512 or 1024 bins (variant 6,
 // stage_pair.hip): a team of VP/256 blocks per z range, the per-row moments
@@ -199,7 +203,7 @@ const char *stage_quad_kernel_name();
 // one launch per step; the new states go to the runs' acc buffers.  mat: some
 // run of the launch has the matter term.
 hipError_t launch_stage_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
-                                   int L, bool uniform, bool mat, hipStream_t s);
+                                   int L, bool uniform, bool mat, bool std_phys, hipStream_t s);
 const char *stage_fast_kernel_name(int VP, int stage);
 // Batched FAST stage over `nitems` work items (run index, first z cell) of
 // runs with the same VP and scheme; `runs` and `items` are device arrays.

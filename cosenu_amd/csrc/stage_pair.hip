@@ -614,7 +614,7 @@ void cosenu_pair_stage(StageArgs A1, StageArgs A2, int L, int nchunks, int nteam
 // nz; every team runs the same L + 9 z steps).  `runs` holds each run's current state
 // in .y and the spare in .acc (the host passes the array with the two
 // swapped after every step).
-template <int NW, int SCHEME, int PAIR, bool UNI>
+template <int NW, int SCHEME, int PAIR, bool UNI, int PHYS = -1>
 __global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int L)
 {
@@ -633,8 +633,9 @@ void cosenu_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int 
         A2.yout = r.acc;
     // a run's last item may be shorter than L (its nz need not be a multiple)
     const int jend = min(it.y + L, r.nz);
-    pair_body<NW, SCHEME, PAIR, UNI, false>(A1, A2, L, it.y, jend, active, ring_a[team],
-                                            ring_b[team], yring + (PAIR == 1 ? team * NW * kYRing : 0));
+    pair_body<NW, SCHEME, PAIR, UNI, false, 1, PHYS>(A1, A2, L, it.y, jend, active, ring_a[team],
+                                                     ring_b[team],
+                                                     yring + (PAIR == 1 ? team * NW * kYRing : 0));
 }
 
 // Rows of 512 or 1024 bins: a team of NB = VP/256 blocks of 8 wavefronts per
@@ -699,13 +700,17 @@ hipError_t launch_wide_nb(const StageArgs &a1, const StageArgs &a2, int pair, in
 
 template <int NW, int SCHEME>
 hipError_t launch_batch_nw(const BatchRun *runs, const int2 *items, int nitems, int L, int pair,
-                           bool uni, hipStream_t s)
+                           bool uni, bool std_phys, hipStream_t s)
 {
     constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
     if (nitems <= 0)
         return hipSuccess;
     dim3 grid((nitems + TPB - 1) / TPB), block(64 * NW * TPB);
-    if (uni && pair == 0)
+    if (uni && SCHEME == 0 && std_phys && pair == 0)
+        hipLaunchKernelGGL((cosenu_pair_batch<NW, SCHEME, 0, true, kStdPhys>), grid, block, 0, s, runs, items, nitems, L);
+    else if (uni && SCHEME == 0 && std_phys)
+        hipLaunchKernelGGL((cosenu_pair_batch<NW, SCHEME, 1, true, kStdPhys>), grid, block, 0, s, runs, items, nitems, L);
+    else if (uni && pair == 0)
         hipLaunchKernelGGL((cosenu_pair_batch<NW, SCHEME, 0, true>), grid, block, 0, s, runs, items, nitems, L);
     else if (uni)
         hipLaunchKernelGGL((cosenu_pair_batch<NW, SCHEME, 1, true>), grid, block, 0, s, runs, items, nitems, L);
@@ -804,17 +809,19 @@ hipError_t launch_stage_pair(const StageArgs &a1, const StageArgs &a2, int pair,
 }
 
 hipError_t launch_stage_pair_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
-                                   int scheme, int L, int pair, bool uniform, hipStream_t s)
+                                   int scheme, int L, int pair, bool uniform, bool std_phys,
+                                   hipStream_t s)
 {
     if (!pair_supported(VP, 0) || L < 1 || (pair != 0 && pair != 1))
         return hipErrorInvalidValue;
+    const bool sp = std_phys;
     switch (VP) {
-    case 64: return scheme == 0 ? launch_batch_nw<2, 0>(runs, items, nitems, L, pair, uniform, s)
-                                : launch_batch_nw<2, 1>(runs, items, nitems, L, pair, uniform, s);
-    case 128: return scheme == 0 ? launch_batch_nw<4, 0>(runs, items, nitems, L, pair, uniform, s)
-                                 : launch_batch_nw<4, 1>(runs, items, nitems, L, pair, uniform, s);
-    default: return scheme == 0 ? launch_batch_nw<8, 0>(runs, items, nitems, L, pair, uniform, s)
-                                : launch_batch_nw<8, 1>(runs, items, nitems, L, pair, uniform, s);
+    case 64: return scheme == 0 ? launch_batch_nw<2, 0>(runs, items, nitems, L, pair, uniform, sp, s)
+                                : launch_batch_nw<2, 1>(runs, items, nitems, L, pair, uniform, sp, s);
+    case 128: return scheme == 0 ? launch_batch_nw<4, 0>(runs, items, nitems, L, pair, uniform, sp, s)
+                                 : launch_batch_nw<4, 1>(runs, items, nitems, L, pair, uniform, sp, s);
+    default: return scheme == 0 ? launch_batch_nw<8, 0>(runs, items, nitems, L, pair, uniform, sp, s)
+                                : launch_batch_nw<8, 1>(runs, items, nitems, L, pair, uniform, sp, s);
     }
 }
 

@@ -405,7 +405,7 @@ void cosenu_quad_stage(StageArgs A, double ca0, double ca1, double ca2, double c
 // item of a launch has the same length L (a run's last item may hold fewer
 // rows), so every team of a block runs the same iterations and barriers.
 // The new state goes to the run's spare buffer (acc).
-template <int NW, bool UNI, bool MAT>
+template <int NW, bool UNI, bool MAT, int PHYS = -1>
 __global__ __launch_bounds__(NW >= 8 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int L)
 {
@@ -419,19 +419,21 @@ void cosenu_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int 
     const BatchRun &r = runs[it.x];
     const StageArgs A = batch_stage_args(r, 1, true);
     const int jend = min(it.y + L, r.nz);
-    quad_body<NW, UNI, MAT, false>(A, r.a[0], r.a[1], r.a[2], r.a[3], r.acc, L, it.y, jend, active,
-                                   rings[team], qring + team * NW * kQRing);
+    quad_body<NW, UNI, MAT, false, PHYS>(A, r.a[0], r.a[1], r.a[2], r.a[3], r.acc, L, it.y, jend,
+                                         active, rings[team], qring + team * NW * kQRing);
 }
 
 template <int NW>
 hipError_t launch_batch_nw(const BatchRun *runs, const int2 *items, int nitems, int L, bool uni,
-                           bool mat, hipStream_t s)
+                           bool mat, bool std_phys, hipStream_t s)
 {
     constexpr int TPB = NW >= 4 ? 1 : 4 / NW;
     if (nitems <= 0)
         return hipSuccess;
     dim3 grid((nitems + TPB - 1) / TPB), block(64 * NW * TPB);
-    if (uni && mat)
+    if (uni && std_phys)
+        hipLaunchKernelGGL((cosenu_quad_batch<NW, true, false, kStdPhys>), grid, block, 0, s, runs, items, nitems, L);
+    else if (uni && mat)
         hipLaunchKernelGGL((cosenu_quad_batch<NW, true, true>), grid, block, 0, s, runs, items, nitems, L);
     else if (uni)
         hipLaunchKernelGGL((cosenu_quad_batch<NW, true, false>), grid, block, 0, s, runs, items, nitems, L);
@@ -526,14 +528,14 @@ hipError_t launch_stage_quad(const StageArgs &a, const double ca[4], double *you
 }
 
 hipError_t launch_stage_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
-                                   int L, bool uniform, bool mat, hipStream_t s)
+                                   int L, bool uniform, bool mat, bool std_phys, hipStream_t s)
 {
     if (L < 1)
         return hipErrorInvalidValue;
     switch (VP) {
-    case 64: return launch_batch_nw<2>(runs, items, nitems, L, uniform, mat, s);
-    case 128: return launch_batch_nw<4>(runs, items, nitems, L, uniform, mat, s);
-    case 256: return launch_batch_nw<8>(runs, items, nitems, L, uniform, mat, s);
+    case 64: return launch_batch_nw<2>(runs, items, nitems, L, uniform, mat, std_phys, s);
+    case 128: return launch_batch_nw<4>(runs, items, nitems, L, uniform, mat, std_phys, s);
+    case 256: return launch_batch_nw<8>(runs, items, nitems, L, uniform, mat, std_phys, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -55,6 +55,7 @@
 // kSync iterations apart), so the waves of a team drift up to kSync - 1
 // iterations apart instead of meeting at every z step.
 #include "fast_math.hpp"
+#include "wide_exchange.hpp"
 
 #include <cstdio>
 
@@ -73,91 +74,8 @@ static_assert(kRing - 5 >= kSync && kSync <= 4, "moment ring too shallow for the
 // [slot = row % 8][field][lane]
 constexpr int kYRing = 8 * 4 * 64;
 
-// Teams of NB > 1 blocks (rows wider than 256 bins, cosenu_pair_wide): each
-// block of 8 wavefronts owns 256 bins of the team's z range, and the moments
-// of a z row are summed over the team's blocks through device memory.  Every
-// wavefront publishes its 6 partials of a row (the values it writes to its
-// block's LDS ring) to a ring in device memory as well, and the partner
-// blocks read them 4 z steps later, when they need the row's totals.
-//
-// A value travels as two 64-bit words, each holding 32 bits of the double
-// and a 32-bit tag (the launch epoch and the z row): a word is written and
-// read with a single agent-scope atomic, so a reader sees either the old or
-// the new word, and a value is complete when both of its words carry the
-// tag it expects.  No fence orders the words, and slots need no clearing
-// between uses or launches (a stale word carries another row or epoch).
-//
-// Blocks form teams in the order they start (a ticket from a 64-bit atomic
-// counter that never resets: launch k of a layout takes tickets k * grid ..
-// (k + 1) * grid - 1), so a waiting block's partners have started already
-// or are the next blocks the dispatcher starts: no block waits on one that
-// cannot get a CU.  The epoch is the launch number k, so graph replays need
-// no memset node; the host zeroes the counter when the layout changes.  A
-// wait is bounded by the wall clock; a block that gives up sets the error
-// word (read by the host at its next synchronisation) and later waits end
-// at once, so a fault cannot hang the device.
-struct WideX {
-    unsigned long long *slots; // [2 rings][nteams][NB][kRing][64 elements][2 words]
-    unsigned long long *ctl;   // [0] ticket, [1] error
-    long long wait_ticks;      // the wait bound in wall-clock ticks (1 s)
-    unsigned tag_flip;         // 0; a test of the give-up path expects tags no block writes
-    unsigned epoch;
-    int team, role, nteams;
-};
-
-struct XWords {
-    unsigned long long w0, w1;
-};
-
-// the partner words of the next rows of the two stages' moment rings
-template <int NB>
-struct XRows {
-    XWords a[NB - 1] = {}, b[NB - 1] = {};
-};
-template <>
-struct XRows<1> {
-};
-
-__device__ inline unsigned xtag(unsigned epoch, int row)
-{
-    return (epoch << 24) | ((unsigned)(row + 64) & 0xFFFFFFu);
-}
-
-__device__ inline void xput(unsigned long long *p, unsigned tag, double v)
-{
-    const unsigned long long t = (unsigned long long)tag << 32;
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    __hip_atomic_store(p, t | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(p + 1, t | (b & 0xFFFFFFFFull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ inline XWords xload(unsigned long long *p)
-{
-    return {__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-            __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
-}
-
-// The value of a slot whose words were loaded into w: re-read until both
-// words carry `tag` (bounded wait, see above).
-__device__ inline double xvalue(XWords w, unsigned long long *p, unsigned tag,
-                                unsigned long long *ctl, long long wait_ticks, bool &gave_up)
-{
-    auto ok = [&] { return (unsigned)(w.w0 >> 32) == tag && (unsigned)(w.w1 >> 32) == tag; };
-    if (!ok() && !gave_up) {
-        if (__hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull)
-            gave_up = true; // an earlier wait timed out: results are void anyway
-        const long long t0 = wall_clock64();
-        while (!gave_up && !ok()) {
-            __builtin_amdgcn_s_sleep(8);
-            w = xload(p);
-            if (!ok() && wall_clock64() - t0 > wait_ticks) {
-                gave_up = true;
-                __hip_atomic_store(ctl + 1, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-    }
-    return __longlong_as_double((long long)(((w.w0 & 0xFFFFFFFFull) << 32) | (w.w1 & 0xFFFFFFFFull)));
-}
+// Teams of NB > 1 blocks (rows wider than 256 bins, cosenu_pair_wide): the
+// per-row moment exchange between a team's blocks is in wide_exchange.hpp.
 
 // A1: the first stage of the pair (1 or 3), A2: the second (2 or 4) of the
 // same run.  PAIR 0: stages 1+2, PAIR 1: stages 3+4.  UNI: no padding bins

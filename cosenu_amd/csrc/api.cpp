@@ -314,7 +314,8 @@ int range_teams(int nz, int VP, int scheme, int bc, int variant, int mult, bool 
 {
     long t = variant == 3   ? (long)stream_resident_waves(scheme, general, stage4) / (VP / 32)
              : variant == 4 ? (long)pair_resident_teams(VP, scheme)
-             : variant == 5 ? (long)quad_resident_teams(VP)
+             : variant == 5 ? (long)(VP > 256 ? quad_wide_resident_blocks() / (VP / 256)
+                                              : quad_resident_teams(VP))
              : variant == 6 ? (long)pair_wide_resident_blocks(scheme) / (VP / 256)
                             : (long)fast_resident_teams(VP, scheme, bc);
     t *= mult;
@@ -362,11 +363,12 @@ struct cosenu_ctx {
     bool stream_general = false; // ... in its general form (nvz > 256 only)
     bool uniform = false;   // nvz == VP, every group of 32 bins of one sign of v
     double *mom[4] = {nullptr, nullptr, nullptr, nullptr}; // its moments of y, A, B, acc
-    // variant 6: the device-memory moment rings of its multi-block teams
-    // (for xteams teams) and the 2 control words (ticket counter, error)
+    // variants 5 and 6 on rows of more than 256 bins: the device-memory
+    // moment rings of their multi-block teams (xwords words) and the 2
+    // control words (ticket counter, error)
     unsigned long long *xslots = nullptr;
     unsigned long long *xctl = nullptr;
-    int xteams = 0;
+    size_t xwords = 0;
     bool wide_give_up_test = false; // COSENU_WIDE_GIVE_UP_TEST: force the error path
     View view;
     bool timing = false;
@@ -475,6 +477,17 @@ bool two_stage_launches(const cosenu_ctx *c)
     return c->p.mode == COSENU_FAST && (c->variant == 4 || c->variant == 6);
 }
 bool four_stage_launch(const cosenu_ctx *c) { return c->p.mode == COSENU_FAST && c->variant == 5; }
+// the launches whose teams span several blocks (rows of more than 256 bins)
+bool wide_teams(const cosenu_ctx *c)
+{
+    return c->p.mode == COSENU_FAST && c->VP > 256 && (c->variant == 5 || c->variant == 6);
+}
+// the words of the moment rings of such a launch
+size_t wide_words(const cosenu_ctx *c)
+{
+    return c->variant == 5 ? quad_wide_slot_words(c->VP, c->nteams)
+                           : pair_wide_slot_words(c->VP, c->nteams);
+}
 // the kernels that write the new state to the spare buffer
 bool swaps_state(const cosenu_ctx *c) { return two_stage_launches(c) || four_stage_launch(c); }
 
@@ -486,31 +499,32 @@ hipError_t launch_pair(const cosenu_ctx *c, int pair, hipStream_t s)
     if (pair == 1)
         a2.yout = c->acc;
     if (c->variant == 6)
-        return c->nteams <= c->xteams
+        return wide_words(c) <= c->xwords
                    ? launch_stage_pair_wide(a1, a2, pair, c->nteams, c->uniform, c->xslots,
                                             c->xctl, c->wide_give_up_test, s)
                    : hipErrorInvalidValue;
     return launch_stage_pair(a1, a2, pair, c->z_chunk, c->nteams, c->uniform, s);
 }
 
-// Variant 6: moment rings for c->nteams teams (allocated or grown here, never
-// inside a graph capture) and zeroed control words.  The rings need no
-// clearing between launches of one team layout (their words are tagged with
-// launch and row); a new layout starts from zeroed rings and a zeroed ticket
-// counter (the kernels derive team, block and launch number from it).
+// Multi-block teams (wide_teams): moment rings for the current layout
+// (allocated or grown here, never inside a graph capture), zeroed, and
+// zeroed control words.  The rings need no clearing between launches of one
+// layout (their words are tagged with launch and row); a layout starts from
+// zeroed rings and a zeroed ticket counter (the kernels derive team, block
+// and launch number from it), as a reset counter numbers launches from 0
+// again.
 int wide_buffers(cosenu_ctx *c)
 {
-    if (c->xteams != c->nteams || !c->xslots) {
-        const size_t words = pair_wide_slot_words(c->VP, c->nteams);
-        if (c->nteams > c->xteams || !c->xslots) {
-            if (c->xslots)
-                (void)hipFree(c->xslots);
-            c->xslots = nullptr;
-            RC_TRY(dev_alloc(c->bytes, &c->xslots, words));
-        }
-        HIP_TRY(hipMemsetAsync(c->xslots, 0, words * sizeof(unsigned long long), c->stream));
-        c->xteams = std::max(c->xteams, c->nteams);
+    const size_t words = wide_words(c);
+    if (words > c->xwords || !c->xslots) {
+        if (c->xslots)
+            (void)hipFree(c->xslots);
+        c->xslots = nullptr;
+        c->xwords = 0;
+        RC_TRY(dev_alloc(c->bytes, &c->xslots, words));
+        c->xwords = words;
     }
+    HIP_TRY(hipMemsetAsync(c->xslots, 0, words * sizeof(unsigned long long), c->stream));
     if (!c->xctl)
         RC_TRY(dev_alloc(c->bytes, &c->xctl, 2));
     HIP_TRY(hipMemsetAsync(c->xctl, 0, 2 * sizeof(unsigned long long), c->stream));
@@ -518,8 +532,8 @@ int wide_buffers(cosenu_ctx *c)
     return 0;
 }
 
-// Variant 6: an error if a block of a launch since the last check gave up
-// waiting for its partners' moments (the launch's results are void).
+// Multi-block teams: an error if a block of a launch since the last check
+// gave up waiting for its partners' moments (the launch's results are void).
 int wide_check(cosenu_ctx *c)
 {
     if (!c->xctl)
@@ -530,8 +544,8 @@ int wide_check(cosenu_ctx *c)
     if (err) {
         const unsigned long long zero = 0;
         HIP_TRY(hipMemcpy(c->xctl + 1, &zero, sizeof(zero), hipMemcpyHostToDevice));
-        return fail(COSENU_EHIP, "two-stage launch of a wide grid: a block waited more than 1 s "
-                                 "for the moments of its team's other blocks; the state is void");
+        return fail(COSENU_EHIP, "launch of a wide grid: a block waited more than 1 s for the "
+                                 "moments of its team's other blocks; the state is void");
     }
     return 0;
 }
@@ -548,6 +562,11 @@ hipError_t launch_quad(const cosenu_ctx *c, hipStream_t s)
     const StageArgs a = stage_args(c, 1);
     const double dt = c->d.dt;
     const double ca[4] = {0.5 * dt, 0.5 * dt, dt, 1.0 / 6.0 * dt};
+    if (c->VP > 256)
+        return wide_words(c) <= c->xwords
+                   ? launch_stage_quad_wide(a, ca, c->acc, c->nteams, c->uniform, c->xslots,
+                                            c->xctl, c->wide_give_up_test, s)
+                   : hipErrorInvalidValue;
     return launch_stage_quad(a, ca, c->acc, c->z_chunk, c->nteams, c->uniform, s);
 }
 
@@ -705,10 +724,17 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             chunk = c->nz % 2 ? 1 : 2;
         } else if (c->VP > 256) {
             // the team kernels hold a team of VP/32 wavefronts in one block
-            // of at most 8: rows of 512 and 1024 bins take two stages per
-            // launch with a team over 2 or 4 blocks, other wide grids the
+            // of at most 8: rows of 512 bins with FD and 3rd-order
+            // dissipation take four stages per launch with a team over 2
+            // blocks (nz = 16384, nvz = 512: 0.484 ms per step against 0.906
+            // ms for two-stage launches; open, nvz = 480: 0.839 against
+            // 0.978), other rows of 512 and 1024 bins two stages per launch
+            // with a team over 2 or 4 blocks, other wide grids the
             // barrier-free kernel
-            c->variant = pair_wide_supported(c->VP, p->bc) && c->nz >= 16 ? 6 : 3;
+            c->variant =
+                quad_wide_supported(c->VP, p->bc, p->scheme, p->flags) && c->nz >= 16 ? 5
+                : pair_wide_supported(c->VP, p->bc) && c->nz >= 16                  ? 6
+                                                                                     : 3;
             chunk = -1;
         } else {
             // FD with 3rd-order dissipation: four stages per launch (nz =
@@ -728,16 +754,19 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
         }
         if (ez != 0)
             chunk = ez;
-        if (c->VP > 256 && !(c->variant == 6 && pair_wide_supported(c->VP, p->bc) && c->nz >= 16))
+        if (c->VP > 256 && !(c->variant == 6 && pair_wide_supported(c->VP, p->bc) && c->nz >= 16) &&
+            !(c->variant == 5 && quad_wide_supported(c->VP, p->bc, p->scheme, p->flags) &&
+              c->nz >= 16))
             c->variant = 3; // the barrier-free kernel takes every wide grid (stream_ok holds)
         if (c->variant == 6 && c->VP <= 256)
             c->variant = 4;
-        if (c->variant == 6 && chunk > 0)
+        if (c->VP > 256 && (c->variant == 5 || c->variant == 6) && chunk > 0)
             chunk = -1; // ranges only
         if (c->variant == 3 && !c->stream_ok)
             c->variant = 2; // shapes the barrier-free kernel does not take
-        if (c->variant == 5 && (!quad_supported(c->VP, p->bc, p->scheme, p->flags) || c->nz < 16))
-            c->variant = 4; // FV, 5th-order dissipation, open boundary
+        if (c->variant == 5 && c->VP <= 256 &&
+            (!quad_supported(c->VP, p->bc, p->scheme, p->flags) || c->nz < 16))
+            c->variant = 4; // FV, 5th-order dissipation
         if (c->variant == 4 && (!pair_supported(c->VP, p->bc) || c->nz < 16))
             c->variant = 2;
         if (chunk < 0) {
@@ -805,7 +834,7 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
         delete c;
         return fail(COSENU_EHIP, std::string("cosenu_create: ") + hipGetErrorString(e));
     }
-    if (c->p.mode == COSENU_FAST && c->variant == 6) {
+    if (wide_teams(c)) {
         c->wide_give_up_test = env_int("COSENU_WIDE_GIVE_UP_TEST", 0) != 0;
         const int rcw = wide_buffers(c);
         if (rcw) {
@@ -1032,15 +1061,16 @@ int cosenu_rhs(cosenu_ctx *c, double *const out[8], ptrdiff_t stride, int off)
         return fail(COSENU_EINVAL, "cosenu_rhs: bad layout");
     HIP_TRY(hipSetDevice(c->p.device));
     RhsArgs r = rhs_args(c, c->y, c->B);
-    // a single RHS of a wide grid: the barrier-free kernel (variant 6 has no
-    // one-stage form)
-    if (c->p.mode == COSENU_FAST && (c->variant == 3 || c->variant == 6)) {
+    // a single RHS of a wide grid: the barrier-free kernel (the multi-block
+    // kernels have no one-stage form)
+    if (c->p.mode == COSENU_FAST && (c->variant == 3 || wide_teams(c))) {
         HIP_TRY(launch_stream_moments(c->y, c->vz, c->mom[0], c->nz, c->nvz, c->VP, c->stream));
         StageArgs a = stage_args(c, 1);
         a.yout = c->B;
         a.mom_in = c->mom[0];
         a.mom_out = nullptr;
-        // variant 6's teams are not the barrier-free kernel's: its own ranges
+        // the multi-block kernels' teams are not the barrier-free kernel's:
+        // its own ranges
         const int nt = c->variant == 3 ? c->nteams
                                        : range_teams(c->nz, c->VP, c->p.scheme, c->p.bc, 3, 1,
                                                      c->stream_general);
@@ -1159,7 +1189,7 @@ const char *cosenu_stage_kernel_name(cosenu_ctx *c, int stage)
     if (c->p.mode == COSENU_FAST && c->variant == 3)
         return stage_stream_kernel_name(stage, c->stream_general);
     if (four_stage_launch(c))
-        return stage == 0 ? stage_quad_kernel_name() : "";
+        return stage != 0 ? "" : c->VP > 256 ? stage_quad_wide_kernel_name() : stage_quad_kernel_name();
     if (two_stage_launches(c) && c->variant == 6)
         return stage < 2 ? stage_pair_wide_kernel_name(stage) : "";
     if (two_stage_launches(c))
@@ -1195,9 +1225,13 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
     for (int var = 2; var <= 6 && !rc; var++)
         for (int L : cands) {
             if (rc || (L > 0 && (L > c->nz || c->nz % L)) ||
-                (var == 3 && !c->stream_ok) || (var != 3 && var != 6 && c->VP > 256) ||
+                (var == 3 && !c->stream_ok) ||
+                (var != 3 && var != 5 && var != 6 && c->VP > 256) ||
                 (var == 4 && (!pair_supported(c->VP, q.bc) || c->nz < 16)) ||
-                (var == 5 && (!quad_supported(c->VP, q.bc, q.scheme, q.flags) || c->nz < 16)) ||
+                (var == 5 && c->VP <= 256 &&
+                 (!quad_supported(c->VP, q.bc, q.scheme, q.flags) || c->nz < 16)) ||
+                (var == 5 && c->VP > 256 &&
+                 (L > 0 || !quad_wide_supported(c->VP, q.bc, q.scheme, q.flags) || c->nz < 16)) ||
                 (var == 6 && (L > 0 || !pair_wide_supported(c->VP, q.bc) || c->nz < 16)))
                 continue;
             c->variant = var;
@@ -1216,7 +1250,7 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
                 if (var == 3)
                     c->nteams4 = range_teams(c->nz, c->VP, q.scheme, q.bc, 3, -L,
                                              c->stream_general, true);
-                if (var == 6 && (rc = wide_buffers(c)) != 0)
+                if (wide_teams(c) && (rc = wide_buffers(c)) != 0)
                     break;
             }
             // whole RK4 steps: the variants move different bytes per stage
@@ -1236,7 +1270,7 @@ int cosenu_autotune(const cosenu_params *p, int *variant, int *z_chunk, double *
                 break;
             }
             ms /= reps;
-            if (var == 6 && (rc = wide_check(c)) != 0)
+            if (wide_teams(c) && (rc = wide_check(c)) != 0)
                 break;
             if (ms < best) {
                 best = ms;

@@ -204,6 +204,17 @@ const char *stage_quad_kernel_name();
 // run of the launch has the matter term.
 hipError_t launch_stage_quad_batch(const BatchRun *runs, const int2 *items, int nitems, int VP,
                                    int L, bool uniform, bool mat, bool std_phys, hipStream_t s);
+// Rows of 512 bins (variant 5 on wide grids): a team of 2 blocks per z
+// range, the moments exchanged as in launch_stage_pair_wide
+// through `slots` (quad_wide_slot_words(VP, nteams) words) and `ctl`.  Range
+// mode only.
+int quad_wide_supported(int VP, int bc, int scheme, int flags);
+size_t quad_wide_slot_words(int VP, int nteams);
+int quad_wide_resident_blocks();
+hipError_t launch_stage_quad_wide(const StageArgs &a, const double ca[4], double *yout,
+                                  int nteams, bool uniform, unsigned long long *slots,
+                                  unsigned long long *ctl, bool give_up_test, hipStream_t s);
+const char *stage_quad_wide_kernel_name();
 const char *stage_fast_kernel_name(int VP, int stage);
 // Batched FAST stage over `nitems` work items (run index, first z cell) of
 // runs with the same VP and scheme; `runs` and `items` are device arrays.

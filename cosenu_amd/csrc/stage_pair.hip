@@ -786,16 +786,9 @@ hipError_t launch_stage_pair_wide(const StageArgs &a1, const StageArgs &a2, int 
         return hipErrorInvalidValue;
     if (uniform && a1.nvz != a1.VP)
         return hipErrorInvalidValue;
-    // the wait bound: 1 s of the device's constant-rate wall clock
-    int dev = 0, khz = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
-        khz = 100000;
-    // give_up_test: every wait looks for a tag no block writes and gives up
-    // after 10 us, so that the error path runs with every kernel still
-    // ending normally
-    const long long ticks = give_up_test ? khz / 100 : 1000LL * khz;
-    const unsigned flip = give_up_test ? 0x80000000u : 0u;
+    long long ticks = 0;
+    unsigned flip = 0;
+    wide_wait_bound(give_up_test, &ticks, &flip);
     if (a1.VP == 512)
         return a1.scheme == 0
                    ? launch_wide_nb<0, 2>(a1, a2, pair, nteams, uniform, slots, ctl, ticks, flip, s)

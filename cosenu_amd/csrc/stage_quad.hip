@@ -44,6 +44,7 @@
 // -2 and -1 when it is produced, and above the grid a ghost row is
 // extrapolated from the two rows before it in the stage's window.
 #include "fast_math.hpp"
+#include "wide_exchange.hpp"
 
 #include <cstdio>
 #include <type_traits>
@@ -73,18 +74,20 @@ constexpr int kQRing = (kTRing + kYRing) * 4 * 64; // doubles per wavefront: [sl
 // without it need not hold).  OPENB: the open boundary.  PHYS: the physics
 // switches (kFlag* bits) fixed at compile time, or -1 to read them from the
 // arguments (nz = 65536 x 128: 2% faster with them fixed).  ca[0..3]: dt/2,
-// dt/2, dt, dt/6.
-template <int NW, bool UNI, bool MAT, bool OPENB, int PHYS = -1>
+// dt/2, dt, dt/6.  NB: blocks per team (1: the whole team in this block;
+// > 1: this block holds bins [X.role * NW * 32, (X.role + 1) * NW * 32) of
+// the team's rows, and the moments are summed over the team's blocks).
+template <int NW, bool UNI, bool MAT, bool OPENB, int PHYS = -1, int NB = 1>
 __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, const double ca1,
                                           const double ca2, const double ca3, double *yout, int L,
                                           int j0, int jend, bool active, double (*rings)[kRing][NW][8],
-                                          double *qring)
+                                          double *qring, const WideX &X = WideX{})
 // rings: [4][kRing][NW][8] (stage s input moments); qring: the team's NW x kQRing
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wit = wave % NW;
     const int h = lane >> 5;
-    const int v = wit * 32 + (lane & 31);
+    const int v = (NB > 1 ? X.role * NW * 32 : 0) + wit * 32 + (lane & 31);
     const int VP = A.VP, nz = A.nz;
     const size_t zs = (size_t)8 * VP;
     const double sig = h ? -1.0 : 1.0;
@@ -131,16 +134,61 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
     const double mudv = A.mu * A.dv;
     const double st = A.st * A.pmo, ct = A.ct * A.pmo, st2 = 2.0 * st, ct2 = 2.0 * ct;
 
-    // partial moments of 4 field values of this lane into ring[row % kRing]
-    auto moments = [&](double (*ring)[NW][8], int row, double p, double q, double r, double s) {
+    // NB > 1: the device-memory slots of ring rg (stage rg + 1's input
+    // moments), block b, z row `row` (32-bit word offsets: the rings of a
+    // launch hold at most a few million words)
+    const unsigned xteam = (unsigned)X.team * (NB * kRing * 128);
+    const unsigned xring = (unsigned)X.nteams * (NB * kRing * 128);
+    auto xslot = [&](int rg, int b, int row) -> unsigned long long * {
+        return X.slots + (xteam + (unsigned)rg * xring + (unsigned)b * (kRing * 128) +
+                          (unsigned)(row & (kRing - 1)) * 128u);
+    };
+    bool gave_up = false;
+    // partial moments of 4 field values of this lane into rings[rg][row %
+    // kRing] (and, NB > 1, into this block's device-memory ring rg)
+    auto moments = [&](int rg, int row, double p, double q, double r, double s) {
         const double S = live ? s : 0.0;
         const double R = live ? sig * r : 0.0;
         const double D = live ? sig * (p - q) : 0.0;
         const double m6[6] = {S, vi * S, R, vi * R, D, vi * D};
         const double t = wave_sum6(m6, lane);
         const int idx = (kMomentLane >> (4 * (lane >> 3))) & 15;
-        if ((lane & 7) == 0 && idx < 6)
-            ring[row & (kRing - 1)][wit][idx] = t;
+        if ((lane & 7) == 0 && idx < 6) {
+            rings[rg][row & (kRing - 1)][wit][idx] = t;
+            if (NB > 1)
+                xput(xslot(rg, X.role, row) + 2 * (wit * 8 + idx), xtag(X.epoch, row), t);
+        }
+    };
+    // NB > 1: this lane's element of a row's partials summed over the team's
+    // blocks, its own block's from LDS (`own`), partner k's (block role ^ k,
+    // k = 1 .. NB-1) from the words w[k-1] loaded earlier.  The sum is the
+    // tree (own + p1) + (p2 + p3): as IEEE addition commutes, every block of
+    // the team rounds it alike, (P0 + P1) + (P2 + P3) in block numbers.
+    const bool xlane = (lane & 7) < 6; // elements 6, 7 of a wavefront's row are unused
+    auto xload_row = [&](XWords *w, int rg, int row) {
+#pragma unroll
+        for (int k = 1; k < NB; k++)
+            w[k - 1] = xlane ? xload(xslot(rg, X.role ^ k, row) + 2 * lane) : XWords{0ull, 0ull};
+    };
+    auto xsum = [&](double own, const XWords *w, int rg, int row) {
+        double p[NB];
+        p[0] = own;
+#pragma unroll
+        for (int k = 1; k < NB; k++)
+            p[k] = xlane ? xvalue(w[k - 1], xslot(rg, X.role ^ k, row) + 2 * lane,
+                                  xtag(X.epoch, row) ^ X.tag_flip, X.ctl, X.wait_ticks, gave_up)
+                         : 0.0;
+        if constexpr (NB == 4)
+            return (p[0] + p[1]) + (p[2] + p[3]);
+        else if constexpr (NB == 2)
+            return p[0] + p[1];
+        else
+            return p[0];
+    };
+    // NB > 1: is row r's total needed by stage s + 1 (its cells, see below)
+    auto xneed = [&](int s, int r) {
+        const int m = 6 - 2 * s;
+        return coll && active && r >= j0 - m && r < jend + m && !(OPENB && (r < 0 || r >= nz));
     };
     // right-hand side at the window centre of z row r: mx = this lane's
     // moment_partial of the row's ring slot, hmr = Hm[r] (MAT); the terms in
@@ -212,9 +260,14 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         }
     }
     if (coll) {
-        moments(rings[0], ibeg, w0[0][2], w0[1][2], w0[2][2], w0[3][2]);
-        moments(rings[0], ibeg + 1, w0[0][3], w0[1][3], w0[2][3], w0[3][3]);
+        moments(0, ibeg, w0[0][2], w0[1][2], w0[2][2], w0[3][2]);
+        moments(0, ibeg + 1, w0[0][3], w0[1][3], w0[2][3], w0[3][3]);
     }
+    // NB > 1: the partners' words of the rows the four stages take next
+    // (ring s, row i - 2 s: published 2 iterations before it is needed),
+    // loaded one iteration ahead, right after the previous row's words were
+    // used
+    XRingRows<4, NB> xw; // (empty for NB = 1)
     double nxt[4];
     load4(A.yin, ibeg + 3, nxt); // the row that enters the window after iteration ibeg
     // The prologue's loads have landed before the loop (the wait-count
@@ -240,6 +293,19 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             mx[2] = moment_partial<NW>(rings[2][c3 & (kRing - 1)], lane);
             mx[3] = moment_partial<NW>(rings[3][c4 & (kRing - 1)], lane);
         }
+        // NB > 1: stage r + 1's totals at its cell c, summed right before
+        // the stage (the partners published row c at stage r of their
+        // previous iteration, or, r = 0, at its start), and the words of its
+        // next cell loaded (1.25 iterations after they were published)
+        auto xstage = [&](auto rc, int c) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            if constexpr (NB > 1) {
+                if (xneed(r, c))
+                    mx[r] = xsum(mx[r], xw.r[r], r, c);
+                if (xneed(r, c + 1))
+                    xload_row(xw.r[r], r, c + 1);
+            }
+        };
         // the matter potential of the four stages' rows, loaded ahead of use
         double hmv[4] = {0.0, 0.0, 0.0, 0.0};
         if (MAT && mat) {
@@ -252,7 +318,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         double nn[4];
         load4(A.yin, i + 4, nn);
         if (coll)
-            moments(rings[0], i + 2, w0[0][top0], w0[1][top0], w0[2][top0], w0[3][top0]);
+            moments(0, i + 2, w0[0][top0], w0[1][top0], w0[2][top0], w0[3][top0]);
 
         // OPENB: a stage's output at a cell above the grid extrapolates from
         // the two rows before it in the stage's window (slots (T+4)%5 and
@@ -270,6 +336,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             }
         };
         // stage 1 at c1: Y2 = y + (dt/2) K1
+        xstage(Rot<0>{}, c1);
         double o[4] = {0.0, 0.0, 0.0, 0.0};
         if (c1 >= j0 - 6 && c1 < jend + 6 && !(OPENB && (c1 < 0 || c1 >= nz))) {
             double K[4];
@@ -278,7 +345,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             for (int f = 0; f < 4; f++)
                 o[f] = fma(ca0, K[f], w0[f][cen0]);
             if (coll)
-                moments(rings[1], c1, o[0], o[1], o[2], o[3]);
+                moments(1, c1, o[0], o[1], o[2], o[3]);
         }
         if (OPENB) {
             ghost(w1, c1, o);
@@ -289,6 +356,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         }
         // stage 2 at c2: Y3; Q and y to the rings.  y of this cell is the y
         // window's oldest row (shifted out at the end of the iteration).
+        xstage(Rot<1>{}, c2);
         if (c2 >= j0 - 4 && c2 < jend + 4 && !(OPENB && (c2 < 0 || c2 >= nz))) {
             double K[4];
             rhs(w1, Rot<T + 1>{}, mx[1], hmv[1], K);
@@ -300,7 +368,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 yslot(c2, f) = y2;
             }
             if (coll)
-                moments(rings[2], c2, o[0], o[1], o[2], o[3]);
+                moments(2, c2, o[0], o[1], o[2], o[3]);
         } else {
 #pragma unroll
             for (int f = 0; f < 4; f++)
@@ -314,6 +382,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 w2[f][old] = o[f];
         }
         // stage 3 at c3: Y4 = y + dt K3; T = y + (Q + (Y4 - y))/3 replaces Q
+        xstage(Rot<2>{}, c3);
         if (c3 >= j0 - 2 && c3 < jend + 2 && !(OPENB && (c3 < 0 || c3 >= nz))) {
             double K[4];
             rhs(w2, Rot<T + 1>{}, mx[2], hmv[2], K);
@@ -325,7 +394,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 t = fma(t + (o[f] - yv), 1.0 / 3.0, yv);
             }
             if (coll)
-                moments(rings[3], c3, o[0], o[1], o[2], o[3]);
+                moments(3, c3, o[0], o[1], o[2], o[3]);
         } else {
 #pragma unroll
             for (int f = 0; f < 4; f++)
@@ -339,6 +408,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 w3[f][old] = o[f];
         }
         // stage 4 at c4: y' = T + (dt/6) K4
+        xstage(Rot<3>{}, c4);
         if (c4 >= j0 && c4 < jend) {
             double K[4];
             rhs(w3, Rot<T + 1>{}, mx[3], hmv[3], K);
@@ -399,6 +469,68 @@ void cosenu_quad_stage(StageArgs A, double ca0, double ca1, double ca2, double c
         quad_body<NW, UNI, MAT, OPENB, PHYS>(A, ca0, ca1, ca2, ca3, yout, L, j0, j0 + L, active,
                                              rings[team], qr);
     }
+}
+
+// Rows of 512 bins: a team of NB = 2 blocks of 8 wavefronts per contiguous z
+// range (range mode only: the grid is nteams * NB blocks, every team the same
+// trip count).  Teams form by ticket (WideX).  (Teams of 4 blocks, for rows
+// of 1024 bins, would hold 3 partners' words of 4 rings, 48 more VGPRs: they
+// spill, and those rows take the two-stage kernel.)
+template <bool UNI, bool MAT, bool OPENB, int NB, int PHYS = -1>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
+void cosenu_quad_wide(StageArgs A, double ca0, double ca1, double ca2, double ca3, double *yout,
+                      int nteams, unsigned long long *slots, unsigned long long *ctl,
+                      long long wait_ticks, unsigned tag_flip)
+{
+    constexpr int NW = 8;
+    __shared__ double rings[4][kRing][NW][8];
+    __shared__ double qring[NW * kQRing];
+    __shared__ unsigned long long ticket;
+    if (threadIdx.x == 0)
+        ticket = __hip_atomic_fetch_add(ctl, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const unsigned long long grid = gridDim.x;
+    const int t = (int)(ticket % grid);
+    WideX X;
+    X.slots = slots;
+    X.ctl = ctl;
+    X.wait_ticks = wait_ticks;
+    X.tag_flip = tag_flip;
+    X.epoch = (unsigned)(ticket / grid);
+    X.team = t / NB;
+    X.role = t % NB;
+    X.nteams = nteams;
+    const bool active = X.team < nteams;
+    const int L = (A.nz + nteams - 1) / nteams;
+    const int j0 = active ? (int)(((long)X.team * A.nz) / nteams) : 0;
+    const int j1 = active ? (int)(((long)(X.team + 1) * A.nz) / nteams) : 0;
+    quad_body<NW, UNI, MAT, OPENB, PHYS, NB>(A, ca0, ca1, ca2, ca3, yout, L, j0, j1, active, rings,
+                                             qring, X);
+}
+
+template <int NB>
+hipError_t launch_wide_nb(const StageArgs &a, const double ca[4], double *yout, int nteams,
+                          bool uni, unsigned long long *slots, unsigned long long *ctl,
+                          long long ticks, unsigned flip, hipStream_t s)
+{
+    dim3 grid(nteams * NB), block(512);
+    const bool mat = (a.flags & kFlagMat) != 0;
+    if (a.bc == 1 && mat) // the open boundary: the general bin layout
+        hipLaunchKernelGGL((cosenu_quad_wide<false, true, true, NB>), grid, block, 0, s, a, ca[0],
+                           ca[1], ca[2], ca[3], yout, nteams, slots, ctl, ticks, flip);
+    else if (a.bc == 1)
+        hipLaunchKernelGGL((cosenu_quad_wide<false, false, true, NB>), grid, block, 0, s, a, ca[0],
+                           ca[1], ca[2], ca[3], yout, nteams, slots, ctl, ticks, flip);
+    else if (uni && a.flags == kStdPhys)
+        hipLaunchKernelGGL((cosenu_quad_wide<true, false, false, NB, kStdPhys>), grid, block, 0, s,
+                           a, ca[0], ca[1], ca[2], ca[3], yout, nteams, slots, ctl, ticks, flip);
+    else if (uni)
+        hipLaunchKernelGGL((cosenu_quad_wide<true, true, false, NB>), grid, block, 0, s, a, ca[0],
+                           ca[1], ca[2], ca[3], yout, nteams, slots, ctl, ticks, flip);
+    else
+        hipLaunchKernelGGL((cosenu_quad_wide<false, true, false, NB>), grid, block, 0, s, a, ca[0],
+                           ca[1], ca[2], ca[3], yout, nteams, slots, ctl, ticks, flip);
+    return hipGetLastError();
 }
 
 // Batched runs: work item g = (run, first z cell) of a packed list; every
@@ -553,6 +685,49 @@ int quad_resident_teams(int VP)
 const char *stage_quad_kernel_name()
 {
     return "cosenu_quad_stage<*, *>";
+}
+
+int quad_wide_supported(int VP, int bc, int scheme, int flags)
+{
+    return VP == 512 && (bc == 0 || bc == 1) && scheme == 1 && !(flags & kFlagKO5);
+}
+
+size_t quad_wide_slot_words(int VP, int nteams)
+{
+    return (size_t)4 * nteams * (VP / 256) * kRing * 64 * 2;
+}
+
+int quad_wide_resident_blocks()
+{
+    int nb = 0, dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&cosenu_quad_wide<false, true, false, 2>), 512, 0) !=
+        hipSuccess)
+        return 0;
+    return nb * prop.multiProcessorCount;
+}
+
+hipError_t launch_stage_quad_wide(const StageArgs &a, const double ca[4], double *yout,
+                                  int nteams, bool uniform, unsigned long long *slots,
+                                  unsigned long long *ctl, bool give_up_test, hipStream_t s)
+{
+    if (!quad_wide_supported(a.VP, a.bc, a.scheme, a.flags) || a.nz < 16 || a.nz > (1 << 23) ||
+        nteams < 1 || nteams > a.nz || !yout || !slots || !ctl)
+        return hipErrorInvalidValue;
+    if (uniform && a.nvz != a.VP)
+        return hipErrorInvalidValue;
+    long long ticks = 0;
+    unsigned flip = 0;
+    wide_wait_bound(give_up_test, &ticks, &flip);
+    return launch_wide_nb<2>(a, ca, yout, nteams, uniform, slots, ctl, ticks, flip, s);
+}
+
+const char *stage_quad_wide_kernel_name()
+{
+    return "cosenu_quad_wide<*>";
 }
 
 } // namespace cosenu

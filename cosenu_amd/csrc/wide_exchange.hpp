@@ -54,6 +54,15 @@ template <>
 struct XRows<1> {
 };
 
+// the partner words of the next rows of NR moment rings
+template <int NR, int NB>
+struct XRingRows {
+    XWords r[NR][NB - 1] = {};
+};
+template <int NR>
+struct XRingRows<NR, 1> {
+};
+
 __device__ inline unsigned xtag(unsigned epoch, int row)
 {
     return (epoch << 24) | ((unsigned)(row + 64) & 0xFFFFFFu);
@@ -93,6 +102,19 @@ __device__ inline double xvalue(XWords w, unsigned long long *p, unsigned tag,
         }
     }
     return __longlong_as_double((long long)(((w.w0 & 0xFFFFFFFFull) << 32) | (w.w1 & 0xFFFFFFFFull)));
+}
+
+// Host: the wait bound, 1 s of the device's constant-rate wall clock.
+// give_up_test: every wait looks for a tag no block writes and gives up after
+// 10 us, so that the error path runs with every kernel still ending normally.
+inline void wide_wait_bound(bool give_up_test, long long *ticks, unsigned *flip)
+{
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        khz = 100000;
+    *ticks = give_up_test ? khz / 100 : 1000LL * khz;
+    *flip = give_up_test ? 0x80000000u : 0u;
 }
 
 } // namespace fastk

@@ -74,20 +74,25 @@ def test_invalid_parameters_are_returned_before_any_device_call(kw, match):
 @pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
 @pytest.mark.parametrize("nvz", [300, 480, 512, 1000])
 @pytest.mark.parametrize("bc", [0, 1])
-def test_fast_mode_takes_wide_grids_with_either_boundary(nvz, bc):
-    """Rows wider than 256 bins take the multi-block two-stage kernel (rows
-    of 512 or 1024 bins) or the barrier-free kernel, in its general form
-    where bin groups straddle v = 0, bins are padded or the boundary is open:
-    validation accepts every such grid.  Without a GPU the call then fails at
-    the first device call, not at validation."""
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_fast_mode_takes_wide_grids_with_either_boundary(nvz, bc, scheme):
+    """Rows wider than 256 bins take the multi-block four-stage kernel (FD
+    with 3rd-order dissipation on rows of 512 bins), the multi-block
+    two-stage kernel (other rows of 512 or 1024 bins) or the barrier-free
+    kernel, in its general form where bin groups straddle v = 0, bins are
+    padded or the boundary is open: validation accepts every such grid.
+    Without a GPU the call then fails at the first device call, not at
+    validation."""
     import cosenu_amd as C
     try:
-        sim = C.Simulation(C.RunParams(nz=64, nvz=nvz, bc=bc, mode=C.FAST))
+        sim = C.Simulation(C.RunParams(nz=64, nvz=nvz, bc=bc, scheme=scheme, mode=C.FAST))
     except C.CosenuError as e:
         assert "hipSetDevice" in str(e) or "hip" in str(e).lower(), str(e)
         assert "FAST mode" not in str(e)
     else:
-        assert sim.info()["fast_variant"] == (6 if nvz in (480, 512, 1000) else 3)
+        expect = (5 if scheme == 1 and nvz in (480, 512)
+                  else 6 if nvz in (480, 512, 1000) else 3)
+        assert sim.info()["fast_variant"] == expect
         sim.close()
 
 

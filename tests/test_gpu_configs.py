@@ -110,11 +110,13 @@ def test_production_grid(gpu, scheme):
 
 @pytest.mark.parametrize("name,scheme,variant", [("c1_open", C.FV, 4), ("c1_open", C.FD, 5),
                                                   ("wide_open", C.FV, 3), ("wide_open", C.FD, 3),
-                                                  ("wide_open", C.FV, 6), ("wide_open", C.FD, 6)])
+                                                  ("wide_open", C.FV, 6), ("wide_open", C.FD, 6),
+                                                  ("wide_open", C.FD, 5)])
 def test_open_boundary_full_size(gpu, monkeypatch, name, scheme, variant):
     """The open boundary at full size: nz = 65536 x 128 (two- and four-stage
     kernels) and nz = 16384 x 480 (a bin group across v = 0 and padded rows:
-    the general barrier-free kernel and the multi-block two-stage kernel).
+    the general barrier-free kernel and the multi-block two- and four-stage
+    kernels).
     FAST against the oracle (moment form, 3 steps) and against STRICT (50
     steps); STRICT's first step equal to the oracle's reference arithmetic."""
     monkeypatch.setenv("COSENU_FAST_VARIANT", str(variant))
@@ -148,37 +150,38 @@ def test_open_boundary_full_size(gpu, monkeypatch, name, scheme, variant):
     assert np.isfinite(f50).all() and not np.array_equal(f50, s0)
 
 
-@pytest.mark.parametrize("scheme", [C.FV, C.FD])
-def test_wide_grid_full_size(gpu, monkeypatch, scheme):
+@pytest.mark.parametrize("scheme,variant", [(C.FV, "6"), (C.FD, "5"), (C.FD, "6")])
+def test_wide_grid_full_size(gpu, monkeypatch, scheme, variant):
     """nz = 16384 x 512 at full size (the `wide` workload), periodic: the
-    multi-block two-stage kernel (variant 6, the default for rows of 512
-    bins) against the oracle (moment form, 3 steps) and against the
+    multi-block kernels (four stages per launch, variant 5, the default for
+    FD on rows of 512 bins; two stages per launch, variant 6, the default
+    for FV) against the oracle (moment form, 3 steps) and against the
     barrier-free kernel (variant 3, 50 steps); total trace conserved."""
-    tag = f"wide {'FV' if scheme == C.FV else 'FD'}"
+    tag = f"wide {'FV' if scheme == C.FV else 'FD'} variant {variant}"
     wl = workload("wide", mode=C.FAST, scheme=scheme)
     p = wl.params
     po = _orc(p)
     a, b, G1, bG1 = wl.factors()
     s0 = a[:, :, None] * b[:, None, :]
     outs = {}
-    for variant in ("6", "3"):
-        monkeypatch.setenv("COSENU_FAST_VARIANT", variant)
-        _log(f"{tag}: nz={p.nz} nvz={p.nvz} periodic, variant {variant}, 3 + 47 steps")
+    for v in (variant, "3"):
+        monkeypatch.setenv("COSENU_FAST_VARIANT", v)
+        _log(f"{tag}: nz={p.nz} nvz={p.nvz} periodic, variant {v}, 3 + 47 steps")
         with _start(p, a, b, G1, bG1) as sim:
-            assert sim.info()["fast_variant"] == int(variant)
+            assert sim.info()["fast_variant"] == int(v)
             d0 = sim.diagnostics()
             sim.step(3)
             f3 = sim.download()
             sim.step(47)
             f50 = sim.download()
             d50 = sim.diagnostics()
-        outs[variant] = (f3, f50, d0, d50)
+        outs[v] = (f3, f50, d0, d50)
     _log(f"{tag}: oracle, moment form 3 steps")
     ref3 = O.step(po, s0, 3, moment_form=True)
-    f3, f50, d0, d50 = outs["6"]
+    f3, f50, d0, d50 = outs[variant]
     r3, r50 = _rel(f3, ref3), _rel(f50, outs["3"][1])
     tr = abs(d50["trace"] - d0["trace"]) / abs(d0["trace"])
-    _log(f"{tag}: variant 6 - oracle after 3 steps {r3:.3e}; variant 6 - variant 3 after 50 "
+    _log(f"{tag}: oracle after 3 steps {r3:.3e}; against variant 3 after 50 "
          f"steps {r50:.3e}; trace drift {tr:.3e}")
     assert r3 <= 1e-10 and r50 <= 1e-12 and tr <= 1e-14
     assert d50["nonfinite"] == 0 and not np.array_equal(f50, s0)

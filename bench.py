@@ -62,7 +62,9 @@ BYTES_PER_CELL_STEP_DEF = sum(ALG_STAGE_BYTES)
 #      slots are the two launches
 #   5 (FD: four stages per launch; stage outputs stay in registers, the
 #      increment sum in LDS): read y, write the new state (64+64), plus the
-#      halo rows each team reads again (12 rows per team range)
+#      halo rows each team reads again (12 rows per team range); on rows of
+#      512 bins a team spans 2 blocks, which exchange the four stages'
+#      moment partials (under 4 bytes per cell)
 #   6 (rows of 512 or 1024 bins, two stages per launch with a team over 2 or
 #      4 blocks): as 4; the per-row moment partials the blocks exchange
 #      (768 bytes per row and block, each way) are under 2 bytes per cell

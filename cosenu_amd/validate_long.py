@@ -89,8 +89,11 @@ def main(argv=None):
         history.append(rec)
         print(json.dumps({"progress": rec}), flush=True)
     last = history[-1]
+    fast = sims["fast"]
     out = {"workload": args.workload, "scheme": args.scheme, "nz": p.nz, "nvz": p.nvz,
            "z": [p.z0, p.z1],
+           "fast_variant": fast.info()["fast_variant"], "fast_kernel": fast.stage_kernel_name(0),
+           "range_teams": fast.info()["range_teams"],
            "steps": args.steps, "pass_1e-10": bool(last["pee_linf"] <= 1e-10 and last["pee_l2"] <= 1e-10
                                                    and last["trace_rel_diff"] <= 1e-10
                                                    and last["lepton_rel_diff"] <= 1e-10),

@@ -88,10 +88,13 @@ typedef struct cosenu_info {
                             3 = barrier-free (default for small periodic
                             problems), 2 = one stage per launch, two lanes
                             per bin (nu / nubar halves; nz < 16); 0 in STRICT.
-                            nvz > 256: 6 = two RK4 stages per launch with a
-                            team of 2 or 4 blocks per z range (default for
-                            rows of 512 or 1024 bins, i.e. nvz 449-512 and
-                            961-1024, either boundary), else 3, in a general
+                            nvz > 256: 5 = all four stages per launch with a
+                            team of 2 blocks per z range (default for FD with
+                            3rd-order dissipation on rows of 512 bins, i.e.
+                            nvz 449-512, either boundary), 6 = two RK4 stages
+                            per launch with a team of 2 or 4 blocks per z
+                            range (default for other rows of 512 or 1024
+                            bins, nvz 961-1024), else 3, in a general
                             form where a group of 32 bins straddles v = 0,
                             bins are padded (nvz not a multiple of 64) or the
                             boundary is open */

@@ -145,7 +145,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
     };
     bool gave_up = false;
     // partial moments of 4 field values of this lane into rings[rg][row %
-    // kRing] (and, NB > 1, into this block's device-memory ring rg)
+    // kRing]; returns the lane's total (wave_sum6)
     auto moments = [&](int rg, int row, double p, double q, double r, double s) {
         const double S = live ? s : 0.0;
         const double R = live ? sig * r : 0.0;
@@ -153,10 +153,24 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         const double m6[6] = {S, vi * S, R, vi * R, D, vi * D};
         const double t = wave_sum6(m6, lane);
         const int idx = (kMomentLane >> (4 * (lane >> 3))) & 15;
-        if ((lane & 7) == 0 && idx < 6) {
+        if ((lane & 7) == 0 && idx < 6)
             rings[rg][row & (kRing - 1)][wit][idx] = t;
-            if (NB > 1)
-                xput(xslot(rg, X.role, row) + 2 * (wit * 8 + idx), xtag(X.epoch, row), t);
+        return t;
+    };
+    // NB > 1: publish a row's partials (t: moments' return value) to this
+    // block's device-memory ring rg.  Every lane stores, its 8-lane group's
+    // total (the same value in all 8 lanes) to the group's element; the two
+    // groups holding duplicates write element 7, which no block reads.  The
+    // exchange's loads and stores are issued on every z step, whatever the
+    // row (a row no stage needs publishes zeros, and its slot is rewritten
+    // before a needed row's words are read from it): the compiler's
+    // in-order count of outstanding memory operations then holds on every
+    // path, and a wait for a partner word does not also wait for the
+    // step's own window loads.
+    auto xpub = [&](int rg, int row, double t) {
+        if constexpr (NB > 1) {
+            const int idx = (kMomentLane >> (4 * (lane >> 3))) & 15;
+            xput(xslot(rg, X.role, row) + 2 * (wit * 8 + idx), xtag(X.epoch, row), t);
         }
     };
     // NB > 1: this lane's element of a row's partials summed over the team's
@@ -168,7 +182,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
     auto xload_row = [&](XWords *w, int rg, int row) {
 #pragma unroll
         for (int k = 1; k < NB; k++)
-            w[k - 1] = xlane ? xload(xslot(rg, X.role ^ k, row) + 2 * lane) : XWords{0ull, 0ull};
+            w[k - 1] = xload(xslot(rg, X.role ^ k, row) + 2 * lane);
     };
     auto xsum = [&](double own, const XWords *w, int rg, int row) {
         double p[NB];
@@ -259,9 +273,14 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             w3[f][k] = 0.0;
         }
     }
-    if (coll) {
-        moments(0, ibeg, w0[0][2], w0[1][2], w0[2][2], w0[3][2]);
-        moments(0, ibeg + 1, w0[0][3], w0[1][3], w0[2][3], w0[3][3]);
+    {
+        double t0 = 0.0, t1 = 0.0;
+        if (coll) {
+            t0 = moments(0, ibeg, w0[0][2], w0[1][2], w0[2][2], w0[3][2]);
+            t1 = moments(0, ibeg + 1, w0[0][3], w0[1][3], w0[2][3], w0[3][3]);
+        }
+        xpub(0, ibeg, t0);
+        xpub(0, ibeg + 1, t1);
     }
     // NB > 1: the partners' words of the rows the four stages take next
     // (ring s, row i - 2 s: published 2 iterations before it is needed),
@@ -295,16 +314,20 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         }
         // NB > 1: stage r + 1's totals at its cell c, summed right before
         // the stage (the partners published row c at stage r of their
-        // previous iteration, or, r = 0, at its start), and the words of its
-        // next cell loaded (1.25 iterations after they were published)
-        auto xstage = [&](auto rc, int c) __attribute__((always_inline)) {
+        // previous iteration, or, r = 0, at its start); after the stage, the
+        // words of its next cell are loaded (~1.5 iterations after the
+        // partners published them)
+        auto xsum_at = [&](auto rc, int c) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
             if constexpr (NB > 1) {
                 if (xneed(r, c))
                     mx[r] = xsum(mx[r], xw.r[r], r, c);
-                if (xneed(r, c + 1))
-                    xload_row(xw.r[r], r, c + 1);
             }
+        };
+        auto xnext = [&](auto rc, int c) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            if constexpr (NB > 1)
+                xload_row(xw.r[r], r, c + 1);
         };
         // the matter potential of the four stages' rows, loaded ahead of use
         double hmv[4] = {0.0, 0.0, 0.0, 0.0};
@@ -317,8 +340,12 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         // the next window row
         double nn[4];
         load4(A.yin, i + 4, nn);
-        if (coll)
-            moments(0, i + 2, w0[0][top0], w0[1][top0], w0[2][top0], w0[3][top0]);
+        {
+            double t0 = 0.0;
+            if (coll)
+                t0 = moments(0, i + 2, w0[0][top0], w0[1][top0], w0[2][top0], w0[3][top0]);
+            xpub(0, i + 2, t0);
+        }
 
         // OPENB: a stage's output at a cell above the grid extrapolates from
         // the two rows before it in the stage's window (slots (T+4)%5 and
@@ -336,8 +363,9 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             }
         };
         // stage 1 at c1: Y2 = y + (dt/2) K1
-        xstage(Rot<0>{}, c1);
+        xsum_at(Rot<0>{}, c1);
         double o[4] = {0.0, 0.0, 0.0, 0.0};
+        double tp = 0.0; // the published total of the stage's output row
         if (c1 >= j0 - 6 && c1 < jend + 6 && !(OPENB && (c1 < 0 || c1 >= nz))) {
             double K[4];
             rhs(w0, Rot<T>{}, mx[0], hmv[0], K);
@@ -345,8 +373,10 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             for (int f = 0; f < 4; f++)
                 o[f] = fma(ca0, K[f], w0[f][cen0]);
             if (coll)
-                moments(1, c1, o[0], o[1], o[2], o[3]);
+                tp = moments(1, c1, o[0], o[1], o[2], o[3]);
         }
+        xpub(1, c1, tp);
+        xnext(Rot<0>{}, c1);
         if (OPENB) {
             ghost(w1, c1, o);
         } else {
@@ -356,7 +386,8 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         }
         // stage 2 at c2: Y3; Q and y to the rings.  y of this cell is the y
         // window's oldest row (shifted out at the end of the iteration).
-        xstage(Rot<1>{}, c2);
+        xsum_at(Rot<1>{}, c2);
+        tp = 0.0;
         if (c2 >= j0 - 4 && c2 < jend + 4 && !(OPENB && (c2 < 0 || c2 >= nz))) {
             double K[4];
             rhs(w1, Rot<T + 1>{}, mx[1], hmv[1], K);
@@ -368,12 +399,14 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 yslot(c2, f) = y2;
             }
             if (coll)
-                moments(2, c2, o[0], o[1], o[2], o[3]);
+                tp = moments(2, c2, o[0], o[1], o[2], o[3]);
         } else {
 #pragma unroll
             for (int f = 0; f < 4; f++)
                 o[f] = 0.0;
         }
+        xpub(2, c2, tp);
+        xnext(Rot<1>{}, c2);
         if (OPENB) {
             ghost(w2, c2, o);
         } else {
@@ -382,7 +415,8 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 w2[f][old] = o[f];
         }
         // stage 3 at c3: Y4 = y + dt K3; T = y + (Q + (Y4 - y))/3 replaces Q
-        xstage(Rot<2>{}, c3);
+        xsum_at(Rot<2>{}, c3);
+        tp = 0.0;
         if (c3 >= j0 - 2 && c3 < jend + 2 && !(OPENB && (c3 < 0 || c3 >= nz))) {
             double K[4];
             rhs(w2, Rot<T + 1>{}, mx[2], hmv[2], K);
@@ -394,12 +428,14 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 t = fma(t + (o[f] - yv), 1.0 / 3.0, yv);
             }
             if (coll)
-                moments(3, c3, o[0], o[1], o[2], o[3]);
+                tp = moments(3, c3, o[0], o[1], o[2], o[3]);
         } else {
 #pragma unroll
             for (int f = 0; f < 4; f++)
                 o[f] = 0.0;
         }
+        xpub(3, c3, tp);
+        xnext(Rot<2>{}, c3);
         if (OPENB) {
             ghost(w3, c3, o);
         } else {
@@ -408,7 +444,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 w3[f][old] = o[f];
         }
         // stage 4 at c4: y' = T + (dt/6) K4
-        xstage(Rot<3>{}, c4);
+        xsum_at(Rot<3>{}, c4);
         if (c4 >= j0 && c4 < jend) {
             double K[4];
             rhs(w3, Rot<T + 1>{}, mx[3], hmv[3], K);
@@ -419,6 +455,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                     out[(size_t)f * VP] = fma(ca3, K[f], tslot(c4, f));
             }
         }
+        xnext(Rot<3>{}, c4);
         // the y window advances (OPENB: row i+3 above the grid extrapolates
         // from rows i+2 and i+1)
         const bool ex = OPENB && i + 3 >= nz;
@@ -521,10 +558,8 @@ hipError_t launch_wide_nb(const StageArgs &a, const double ca[4], double *yout, 
     else if (a.bc == 1)
         hipLaunchKernelGGL((cosenu_quad_wide<false, false, true, NB>), grid, block, 0, s, a, ca[0],
                            ca[1], ca[2], ca[3], yout, nteams, slots, ctl, ticks, flip);
-    else if (uni && a.flags == kStdPhys)
-        hipLaunchKernelGGL((cosenu_quad_wide<true, false, false, NB, kStdPhys>), grid, block, 0, s,
-                           a, ca[0], ca[1], ca[2], ca[3], yout, nteams, slots, ctl, ticks, flip);
-    else if (uni)
+    else if (uni) // (the benchmark physics fixed at compile time spills here: 0.508 against
+                  // 0.467 ms per step at nz = 16384, nvz = 512)
         hipLaunchKernelGGL((cosenu_quad_wide<true, true, false, NB>), grid, block, 0, s, a, ca[0],
                            ca[1], ca[2], ca[3], yout, nteams, slots, ctl, ticks, flip);
     else

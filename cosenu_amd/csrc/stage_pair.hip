@@ -158,20 +158,33 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
                           (unsigned)(row & (kRing - 1)) * 128u);
     };
     bool gave_up = false;
-    // partial moments of 4 field values of this lane into ring[row % kRing]
-    // (and, NB > 1, into this block's device-memory ring rg)
-    auto moments = [&](double (*ring)[NW][8], int rg, int row, double p, double q, double r,
-                       double s) {
+    // partial moments of 4 field values of this lane into ring[row % kRing];
+    // returns the lane's total (wave_sum6)
+    auto moments = [&](double (*ring)[NW][8], int row, double p, double q, double r, double s) {
         const double S = live ? s : 0.0;
         const double R = live ? sig * r : 0.0;
         const double D = live ? sig * (p - q) : 0.0;
         const double m6[6] = {S, vi * S, R, vi * R, D, vi * D};
         const double t = wave_sum6(m6, lane);
         const int idx = (kMomentLane >> (4 * (lane >> 3))) & 15;
-        if ((lane & 7) == 0 && idx < 6) {
+        if ((lane & 7) == 0 && idx < 6)
             ring[row & (kRing - 1)][wit][idx] = t;
-            if (NB > 1)
-                xput(xslot(rg, X.role, row) + 2 * (wit * 8 + idx), xtag(X.epoch, row), t);
+        return t;
+    };
+    // NB > 1: publish a row's partials (t: moments' return value) to this
+    // block's device-memory ring rg.  Every lane stores, its 8-lane group's
+    // total (the same value in all 8 lanes) to the group's element; the two
+    // groups holding duplicates write element 7, which no block reads.  The
+    // exchange's loads and stores are issued on every z step, whatever the
+    // row (a row no stage needs publishes zeros, and its slot is rewritten
+    // before a needed row's words are read from it): the compiler's
+    // in-order count of outstanding memory operations then holds on every
+    // path, and a wait for a partner word does not also wait for the
+    // step's own window loads.
+    auto xpub = [&](int rg, int row, double t) {
+        if constexpr (NB > 1) {
+            const int idx = (kMomentLane >> (4 * (lane >> 3))) & 15;
+            xput(xslot(rg, X.role, row) + 2 * (wit * 8 + idx), xtag(X.epoch, row), t);
         }
     };
     // NB > 1: this lane's element of a row's partials summed over the team's
@@ -183,7 +196,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     auto xload_row = [&](XWords *w, int rg, int row) {
 #pragma unroll
         for (int k = 1; k < NB; k++)
-            w[k - 1] = xlane ? xload(xslot(rg, X.role ^ k, row) + 2 * lane) : XWords{0ull, 0ull};
+            w[k - 1] = xload(xslot(rg, X.role ^ k, row) + 2 * lane);
     };
     auto xsum = [&](double own, const XWords *w, int rg, int row) {
         double p[NB];
@@ -277,10 +290,16 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             wb[f][k] = 0.0;
         }
     }
-    if (coll) {
-        moments(ra, 0, ibeg + 1, wa[0][4], wa[1][4], wa[2][4], wa[3][4]);
-        moments(ra, 0, ibeg + 2, wa[0][5], wa[1][5], wa[2][5], wa[3][5]);
-        moments(ra, 0, ibeg + 3, wa[0][6], wa[1][6], wa[2][6], wa[3][6]);
+    {
+        double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+        if (coll) {
+            t1 = moments(ra, ibeg + 1, wa[0][4], wa[1][4], wa[2][4], wa[3][4]);
+            t2 = moments(ra, ibeg + 2, wa[0][5], wa[1][5], wa[2][5], wa[3][5]);
+            t3 = moments(ra, ibeg + 3, wa[0][6], wa[1][6], wa[2][6], wa[3][6]);
+        }
+        xpub(0, ibeg + 1, t1);
+        xpub(0, ibeg + 2, t2);
+        xpub(0, ibeg + 3, t3);
     }
     // The window's first rows have landed before the loop starts.  Without
     // this wait the compiler's wait-count analysis, which merges the loop
@@ -359,8 +378,12 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             if (cell_b)
                 load4(A2.y2, jb, qb); // Q of the cell, written by stages 1+2
         }
-        if (coll)
-            moments(ra, 0, i + 4, wa[0][7], wa[1][7], wa[2][7], wa[3][7]);
+        {
+            double t0 = 0.0;
+            if (coll)
+                t0 = moments(ra, i + 4, wa[0][7], wa[1][7], wa[2][7], wa[3][7]);
+            xpub(0, i + 4, t0);
+        }
         double F[4] = {0.0, 0.0, 0.0, 0.0};
         if (SCHEME == 0 && advect)
             face(wa, F);
@@ -374,12 +397,12 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         if constexpr (NB > 1) {
             if (xa)
                 mxa = xsum(mxa, xr.a, 0, i);
-            if (xneed_a(i + 1))
-                xload_row(xr.a, 0, i + 1);
+            xload_row(xr.a, 0, i + 1);
         }
 
         // first stage at cell i: its output becomes the newest row of wb
         double oa[4] = {0.0, 0.0, 0.0, 0.0};
+        double ta = 0.0; // the published total of the first stage's output row
         if (cell_a && !ghost_a) {
             double K[4], Fl[4], Fr[4];
 #pragma unroll
@@ -396,12 +419,13 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             for (int f = 0; f < 4; f++)
                 oa[f] = fma(A1.a, K[f], PAIR == 0 ? wa[f][3] : ya[f]);
             if (coll)
-                moments(rb, 1, i, oa[0], oa[1], oa[2], oa[3]);
+                ta = moments(rb, i, oa[0], oa[1], oa[2], oa[3]);
         } else if (cell_a && OPENB && i >= nz) {
 #pragma unroll
             for (int f = 0; f < 4; f++)
                 oa[f] = wb[f][7] * 2 - wb[f][6]; // rows i-1, i-2
         }
+        xpub(1, i, ta);
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             if (OPENB)
@@ -426,8 +450,6 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             if constexpr (NB > 1) {
                 if (xb)
                     mxb = xsum(mxb, xr.b, 1, jb);
-                if (xneed_b(jb + 1))
-                    xload_row(xr.b, 1, jb + 1);
             }
             if (cell_b) {
                 double K[4], Gl[4], Gr[4];
@@ -469,10 +491,8 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
                 Fb[f] = G[f];
             }
         }
-        if constexpr (NB > 1) {
-            if (!face_b && xneed_b(jb + 1)) // (a range's first second-stage row has a face)
-                xload_row(xr.b, 1, jb + 1);
-        }
+        if constexpr (NB > 1)
+            xload_row(xr.b, 1, jb + 1);
         // OPENB: row i+5 above the grid extrapolates from rows i+4, i+3, one
         // row after another as the reference fills its ghost cells
         const bool extrap = OPENB && i + 5 >= nz;

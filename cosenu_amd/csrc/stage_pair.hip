@@ -322,8 +322,9 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     // NB > 1: the other blocks' partials of the rows the two stages take next
     // (first stage: row i, published by the partners 4 iterations before it
     // is needed; second stage: row jb, published when their first stage
-    // computed it), loaded one iteration ahead, right after the previous
-    // row's words were used, so that a load has most of an iteration to land
+    // computed it), loaded one iteration ahead: the first stage's right
+    // after its row's words were used, the second's at the end of the
+    // iteration
     XRows<NB> xr; // (empty for NB = 1)
 
     auto iteration = [&](const int i) {

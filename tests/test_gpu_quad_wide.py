@@ -193,3 +193,33 @@ def test_quad_wide_give_up_is_reported(gpu, monkeypatch):
         sim.step(3)
         out = sim.download()
     assert _rel(out, O.step(po, s, 3)) <= 1e-10
+
+
+@pytest.mark.parametrize("nz,bc", [(16, O.PERIODIC), (100, O.PERIODIC), (100, O.OPEN)])
+def test_quad_wide_small_and_ragged_ranges(gpu, nz, bc):
+    """The smallest grid the kernel takes (nz = 16: teams of 2 rows, ranges
+    shorter than a stage's halo) and a z extent that splits into ranges of
+    unequal length: within 1e-10 of the oracle after 13 steps."""
+    po, pc = _quad(nz, 512, bc=bc)
+    _, _, s = seeded_ic(po, 0.6, 0.5, 0.9, seed=78, amp=1e-3)
+    print(f"quad wide small/ragged nz={nz} bc={bc}", flush=True)
+    ref = O.step(po, s, 13)
+    with C.Simulation(pc) as sim:
+        info = sim.info()
+        assert info["fast_variant"] == 5 and info["range_teams"] > 1
+        sim.upload(s)
+        sim.step(13)
+        out = sim.download()
+    print(f"  {info['range_teams']} teams: rel {_rel(out, ref):.3e}", flush=True)
+    assert np.isfinite(out).all()
+    assert _rel(out, ref) <= 1e-10
+
+
+def test_quad_wide_autotune_times_the_wide_kernels(gpu, monkeypatch):
+    """The autotuner times the barrier-free kernel and both multi-block
+    kernels on rows of 512 bins (FD) and records the fastest."""
+    monkeypatch.delenv("COSENU_FAST_VARIANT")
+    variant, chunk, ms = C.autotune(C.RunParams(nz=1024, nvz=512, scheme=C.FD, mode=C.FAST))
+    print(f"autotune FD nz=1024 nvz=512: variant {variant}, chunk {chunk}, {ms:.3f} ms", flush=True)
+    assert variant in (3, 5, 6) and ms > 0
+    assert variant == 3 or chunk < 0  # the multi-block kernels run contiguous ranges only

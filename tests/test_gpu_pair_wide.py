@@ -52,6 +52,10 @@ CASES = [
     (O.FV, O.VAC | O.COLL, O.PERIODIC, 1024),
     (O.FD, O.VAC | O.COLL, O.PERIODIC, 1024),
     (O.FV, O.VAC | O.MAT | O.COLL, O.OPEN, 1000),
+    # the narrowest rows padded to 512 and 1024 bins: one live bin in the
+    # last live group, whole groups of padding
+    (O.FV, O.VAC | O.COLL, O.PERIODIC, 449),
+    (O.FD, O.VAC | O.COLL, O.OPEN, 961),
 ]
 
 

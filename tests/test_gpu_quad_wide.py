@@ -40,9 +40,13 @@ def _rel(a, b):
 
 # (flags, boundary, nvz): uniform rows (512: every group of 32 bins of one
 # sign of v) with the benchmark physics, the matter term, no advection and
-# no collective term; padded rows with a group across v = 0 (480); the open
-# boundary with and without the matter term
+# no collective term; padded rows with a group across v = 0 (480; 449, the
+# narrowest row padded to 512: one live bin in its last live group and a
+# group of padding only; 511); the open boundary with and without the
+# matter term
 CASES = [
+    (O.VAC | O.COLL, O.PERIODIC, 449),
+    (O.VAC | O.COLL, O.OPEN, 511),
     (O.VAC | O.COLL, O.PERIODIC, 512),
     (O.VAC | O.MAT | O.COLL, O.PERIODIC, 512),
     (O.COLL | O.ADVEC_OFF, O.PERIODIC, 512),

@@ -184,8 +184,9 @@ const char *stage_pair_kernel_name(int pair);
 int pair_wide_supported(int VP, int bc);
 size_t pair_wide_slot_words(int VP, int nteams);
 int pair_wide_resident_blocks(int scheme);
-// give_up_test: every wait of the launch gives up at once (a test of the
-// error path; the launch's results are void)
+// give_up_test: every wait of the launch looks for a tag no block writes and
+// gives up after 10 us (a test of the error path; the launch's results are
+// void)
 hipError_t launch_stage_pair_wide(const StageArgs &a1, const StageArgs &a2, int pair, int nteams,
                                   bool uniform, unsigned long long *slots,
                                   unsigned long long *ctl, bool give_up_test, hipStream_t s);

@@ -754,10 +754,18 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
         }
         if (ez != 0)
             chunk = ez;
-        if (c->VP > 256 && !(c->variant == 6 && pair_wide_supported(c->VP, p->bc) && c->nz >= 16) &&
-            !(c->variant == 5 && quad_wide_supported(c->VP, p->bc, p->scheme, p->flags) &&
-              c->nz >= 16))
-            c->variant = 3; // the barrier-free kernel takes every wide grid (stream_ok holds)
+        if (c->VP > 256) {
+            // as for narrower rows, four stages per launch fall back to two
+            // (FV, 5th-order dissipation, rows of 1024 bins); the
+            // barrier-free kernel takes every wide grid (stream_ok holds)
+            const bool qw =
+                quad_wide_supported(c->VP, p->bc, p->scheme, p->flags) && c->nz >= 16;
+            const bool pw = pair_wide_supported(c->VP, p->bc) && c->nz >= 16;
+            if (c->variant == 5 && !qw)
+                c->variant = 6;
+            if ((c->variant == 6 && !pw) || (c->variant != 5 && c->variant != 6))
+                c->variant = 3;
+        }
         if (c->variant == 6 && c->VP <= 256)
             c->variant = 4;
         if (c->VP > 256 && (c->variant == 5 || c->variant == 6) && chunk > 0)

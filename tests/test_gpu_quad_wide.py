@@ -227,3 +227,22 @@ def test_quad_wide_autotune_times_the_wide_kernels(gpu, monkeypatch):
     print(f"autotune FD nz=1024 nvz=512: variant {variant}, chunk {chunk}, {ms:.3f} ms", flush=True)
     assert variant in (3, 5, 6) and ms > 0
     assert variant == 3 or chunk < 0  # the multi-block kernels run contiguous ranges only
+
+
+@pytest.mark.parametrize("forced,scheme,flags,nvz,expect", [
+    ("5", O.FD, O.VAC | O.COLL, 512, 5),
+    ("5", O.FV, O.VAC | O.COLL, 512, 6),            # no four-stage FV: two stages
+    ("5", O.FD, O.VAC | O.COLL | O.KO5, 512, 6),    # nor with 5th-order dissipation
+    ("5", O.FD, O.VAC | O.COLL, 1024, 6),           # nor on rows of 1024 bins
+    ("4", O.FV, O.VAC | O.COLL, 512, 3),            # single-block kernels: barrier-free
+    ("6", O.FD, O.VAC | O.COLL, 300, 3),            # rows of 320 bins: barrier-free
+])
+def test_forced_variant_falls_back_on_wide_rows(gpu, monkeypatch, forced, scheme, flags, nvz,
+                                                expect):
+    """COSENU_FAST_VARIANT on rows wider than 256 bins: a kernel that does not
+    take the grid falls back as on narrower rows (four stages to two, then
+    the barrier-free kernel)."""
+    monkeypatch.setenv("COSENU_FAST_VARIANT", forced)
+    pc = C.RunParams(nz=64, nvz=nvz, scheme=scheme, flags=flags, mode=C.FAST)
+    with C.Simulation(pc) as sim:
+        assert sim.info()["fast_variant"] == expect

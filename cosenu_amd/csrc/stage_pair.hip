@@ -589,21 +589,7 @@ void cosenu_pair_wide(StageArgs A1, StageArgs A2, int nteams, unsigned long long
     __shared__ double ring_a[kRing][NW][8];
     __shared__ double ring_b[kRing][NW][8];
     __shared__ double yring[PAIR == 1 ? NW * kYRing : 1];
-    __shared__ unsigned long long ticket;
-    if (threadIdx.x == 0)
-        ticket = __hip_atomic_fetch_add(ctl, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const unsigned long long grid = gridDim.x;
-    const int t = (int)(ticket % grid);
-    WideX X;
-    X.slots = slots;
-    X.ctl = ctl;
-    X.wait_ticks = wait_ticks;
-    X.tag_flip = tag_flip;
-    X.epoch = (unsigned)(ticket / grid);
-    X.team = t / NB;
-    X.role = t % NB;
-    X.nteams = nteams;
+    const WideX X = wide_join<NB>(slots, ctl, wait_ticks, tag_flip, nteams);
     const bool active = X.team < nteams;
     const int L = (A1.nz + nteams - 1) / nteams;
     const int j0 = active ? (int)(((long)X.team * A1.nz) / nteams) : 0;

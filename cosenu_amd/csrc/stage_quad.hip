@@ -514,7 +514,7 @@ void cosenu_quad_stage(StageArgs A, double ca0, double ca1, double ca2, double c
 // of 1024 bins, would hold 3 partners' words of 4 rings, 48 more VGPRs: they
 // spill 192-300 B per lane and ran 1.9 ms per step against 1.05 ms for the
 // two-stage kernel at nz = 8192, so those rows take the two-stage kernel.)
-template <bool UNI, bool MAT, bool OPENB, int NB, int PHYS = -1>
+template <bool UNI, bool MAT, bool OPENB, int NB>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_quad_wide(StageArgs A, double ca0, double ca1, double ca2, double ca3, double *yout,
                       int nteams, unsigned long long *slots, unsigned long long *ctl,
@@ -523,27 +523,13 @@ void cosenu_quad_wide(StageArgs A, double ca0, double ca1, double ca2, double ca
     constexpr int NW = 8;
     __shared__ double rings[4][kRing][NW][8];
     __shared__ double qring[NW * kQRing];
-    __shared__ unsigned long long ticket;
-    if (threadIdx.x == 0)
-        ticket = __hip_atomic_fetch_add(ctl, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const unsigned long long grid = gridDim.x;
-    const int t = (int)(ticket % grid);
-    WideX X;
-    X.slots = slots;
-    X.ctl = ctl;
-    X.wait_ticks = wait_ticks;
-    X.tag_flip = tag_flip;
-    X.epoch = (unsigned)(ticket / grid);
-    X.team = t / NB;
-    X.role = t % NB;
-    X.nteams = nteams;
+    const WideX X = wide_join<NB>(slots, ctl, wait_ticks, tag_flip, nteams);
     const bool active = X.team < nteams;
     const int L = (A.nz + nteams - 1) / nteams;
     const int j0 = active ? (int)(((long)X.team * A.nz) / nteams) : 0;
     const int j1 = active ? (int)(((long)(X.team + 1) * A.nz) / nteams) : 0;
-    quad_body<NW, UNI, MAT, OPENB, PHYS, NB>(A, ca0, ca1, ca2, ca3, yout, L, j0, j1, active, rings,
-                                             qring, X);
+    quad_body<NW, UNI, MAT, OPENB, -1, NB>(A, ca0, ca1, ca2, ca3, yout, L, j0, j1, active, rings,
+                                           qring, X);
 }
 
 template <int NB>

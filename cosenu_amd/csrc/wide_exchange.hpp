@@ -54,6 +54,31 @@ template <>
 struct XRows<1> {
 };
 
+// A block of a multi-block launch joins its team: a ticket from the launch's
+// counter (ctl[0]) gives team, role and launch epoch.  Contains a barrier:
+// every thread of the block calls it.
+template <int NB>
+__device__ inline WideX wide_join(unsigned long long *slots, unsigned long long *ctl,
+                                  long long wait_ticks, unsigned tag_flip, int nteams)
+{
+    __shared__ unsigned long long ticket;
+    if (threadIdx.x == 0)
+        ticket = __hip_atomic_fetch_add(ctl, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const unsigned long long grid = gridDim.x;
+    const int t = (int)(ticket % grid);
+    WideX X;
+    X.slots = slots;
+    X.ctl = ctl;
+    X.wait_ticks = wait_ticks;
+    X.tag_flip = tag_flip;
+    X.epoch = (unsigned)(ticket / grid);
+    X.team = t / NB;
+    X.role = t % NB;
+    X.nteams = nteams;
+    return X;
+}
+
 // the partner words of the next rows of NR moment rings
 template <int NR, int NB>
 struct XRingRows {

@@ -10,6 +10,14 @@ advances the same initial state in both modes and reports, every --every
 steps and at the end: P_ee(z, v) = ee/ee0 normwise (||d||_inf/||P||_inf and
 L2), the total trace and the lepton number of each mode, and their drift
 from t = 0.  Prints one JSON line at the end.
+
+Where the run turns unstable (the flavour instability grows), any
+difference of rounding grows with it, and two bit-exact evaluations of the
+reference's arithmetic drift apart as well.  --against strict-ulp measures
+that intrinsic spread: it advances, in place of FAST, a second STRICT run
+whose initial state differs from the first by one unit in the last place
+in every value (sign random, seed fixed).  FAST's distance from STRICT is
+then judged against STRICT's distance from itself.
 """
 from __future__ import annotations
 
@@ -26,8 +34,8 @@ from .initial import workload
 
 
 def subsampled(wl, nz: int):
-    """The workload on a periodic domain of nz cells with the same dz, centred
-    on z = 0 (physics, initial-state recipe and seed unchanged)."""
+    """The workload on a domain of nz cells with the same dz, centred on z = 0
+    (boundary, physics, initial-state recipe and seed unchanged)."""
     p = wl.params
     half = 0.5 * nz * (p.z1 - p.z0) / p.nz
     wl.params = dataclasses.replace(p, nz=nz, z0=-half, z1=half)
@@ -41,12 +49,16 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--every", type=int, default=2000)
     ap.add_argument("--nz", type=int, default=0,
-                    help="z-subsampled periodic domain: this many cells at the workload's dz")
+                    help="z-subsampled domain: this many cells at the workload's dz")
+    ap.add_argument("--against", choices=["fast", "strict-ulp"], default="fast",
+                    help="the second run: FAST mode (default), or STRICT mode from the "
+                         "initial state perturbed by one ulp per value")
     args = ap.parse_args(argv)
     scheme = FV if args.scheme == "fv" else FD
 
     sims = {}
-    for name, mode in (("strict", STRICT), ("fast", FAST)):
+    second = FAST if args.against == "fast" else STRICT
+    for name, mode in (("strict", STRICT), ("fast", second)):
         wl = workload(args.workload, mode=mode, scheme=scheme)
         if args.nz:
             wl = subsampled(wl, args.nz)
@@ -55,6 +67,10 @@ def main(argv=None):
         sim.set_profile_v(G1, bG1)
         sim.init_separable(a, b, STATE0)
         sim.init_separable(a, b, STATE)
+        if name == "fast" and args.against == "strict-ulp":
+            y = sim.download()
+            sgn = np.where(np.random.default_rng(12345).random(y.shape) < 0.5, -1.0, 1.0)
+            sim.upload(np.where(y != 0.0, y + sgn * np.spacing(y), y))
         sims[name] = sim
     p = wl.params
     ee0 = sims["strict"].download(STATE0)[0]
@@ -92,6 +108,7 @@ def main(argv=None):
     fast = sims["fast"]
     out = {"workload": args.workload, "scheme": args.scheme, "nz": p.nz, "nvz": p.nvz,
            "z": [p.z0, p.z1],
+           "against": args.against,
            "fast_variant": fast.info()["fast_variant"], "fast_kernel": fast.stage_kernel_name(0),
            "range_teams": fast.info()["range_teams"],
            "steps": args.steps, "pass_1e-10": bool(last["pee_linf"] <= 1e-10 and last["pee_l2"] <= 1e-10

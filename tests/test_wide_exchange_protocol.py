@@ -1,0 +1,185 @@
+"""The multi-block moment exchange of the wide kernels, as a schedule model.
+
+Rows of 512 and 1024 bins split a z range's team over several blocks
+(cosenu_pair_wide in stage_pair.hip, cosenu_quad_wide in stage_quad.hip).
+Every wavefront publishes its partial moments of a row into a device-memory
+ring of kRing rows (slot = row mod kRing, tagged with the row), on every z
+step whatever the row, and the partner blocks' wavefronts read a row's slots
+when a stage needs that row's totals, waiting until the slot carries the
+row's tag.  Nothing but that data dependency (and each block's own barriers)
+orders the blocks, so the design rests on two claims:
+
+* no slot is overwritten with a later row before every partner wavefront
+  that needs the earlier row has read it (a reader would otherwise wait for
+  a tag that never comes back);
+* the waits never form a cycle (every block finishes).
+
+This model replays each kernel's publish / read / barrier sequence per
+wavefront (the kernels' loop bounds, ring depths, barrier spacing and read
+conditions), with wavefronts as independent agents that drift between their
+block's barriers, under random interleavings, on grids at the edges of the
+kernels' range (the shortest z ranges, ragged ranges, the open boundary's
+ghost rows), and checks both claims.  A wavefront's reads of its own block's
+LDS rings are covered by the single-block kernels' own tests and are not
+modelled.
+"""
+import random
+
+import pytest
+
+W = 3  # wavefronts per block (the kernels have 8; 3 already drift independently)
+
+
+def quad_program(j0, jend, L, nz, openb):
+    """One wavefront's action list in the four-stage kernel (quad_body, NB > 1):
+    ('pub', ring, row), ('read', ring, row), ('bar',)."""
+    ibeg, iend = j0 - 6, j0 + L + 6
+
+    def need(s, r):  # xneed: rows whose totals stage s + 1 uses
+        m = 6 - 2 * s
+        return j0 - m <= r < jend + m and not (openb and (r < 0 or r >= nz))
+
+    prog = [("pub", 0, ibeg), ("pub", 0, ibeg + 1)]
+    for i in range(ibeg, iend):
+        if (i - ibeg) % 5 in (0, 2, 4):  # barriers at iterations 0, 2, 4 of each group of 5
+            prog.append(("bar",))
+        prog.append(("pub", 0, i + 2))
+        c = [i, i - 2, i - 4, i - 6]
+        for s in range(4):  # stage s + 1: read ring s at its cell, then publish ring s + 1
+            if need(s, c[s]):
+                prog.append(("read", s, c[s]))
+            if s < 3:
+                prog.append(("pub", s + 1, c[s]))
+    return prog
+
+
+def pair_program(j0, jend, L, nz, openb, fd):
+    """One wavefront's action list in the two-stage kernel (pair_body, NB > 1)."""
+    ibeg, iend = j0 - 5, j0 + L + 4
+    last = ibeg + 2 * ((iend - ibeg + 1) // 2) if fd else iend  # FD: pairs of iterations
+
+    def need_a(r):
+        return j0 - 4 <= r < jend + 4 and not (openb and (r < 0 or r >= nz))
+
+    def need_b(r):
+        return j0 <= r < jend
+
+    prog = [("pub", 0, ibeg + 1), ("pub", 0, ibeg + 2), ("pub", 0, ibeg + 3)]
+    for i in range(ibeg, last):
+        prog.append(("pub", 0, i + 4))
+        if ((i - ibeg) & 3) == 0:  # a barrier every kSync = 4 iterations
+            prog.append(("bar",))
+        if need_a(i):
+            prog.append(("read", 0, i))
+        prog.append(("pub", 1, i))
+        jb = i - 4
+        if need_b(jb):
+            prog.append(("read", 1, jb))
+    return prog
+
+
+def run_team(programs, nb, kring, rng):
+    """Interleave nb blocks x W wavefronts at random.  Returns None when every
+    agent finished, else a description of the failure."""
+    agents = [(b, w) for b in range(nb) for w in range(W)]
+    pc = {a: 0 for a in agents}
+    bar_count = {a: 0 for a in agents}        # barriers passed
+    slot = {}                                 # (block, wave, ring, row % kring) -> row
+    while True:
+        runnable = []
+        for a in agents:
+            prog = programs[a]
+            if pc[a] >= len(prog):
+                continue
+            op = prog[pc[a]]
+            b, _ = a
+            if op[0] == "bar":
+                # passes once every wavefront of its block has reached this barrier
+                if all(bar_count[(b, v)] > bar_count[a] or
+                       (pc[(b, v)] < len(programs[(b, v)]) and
+                        programs[(b, v)][pc[(b, v)]][0] == "bar" and
+                        bar_count[(b, v)] == bar_count[a]) for v in range(W)):
+                    runnable.append(a)
+            elif op[0] == "read":
+                _, ring, row = op
+                ok = True
+                for pb in range(nb):
+                    if pb == b:
+                        continue
+                    for v in range(W):
+                        have = slot.get((pb, v, ring, row % kring))
+                        if have is not None and have > row:
+                            return (f"block {pb} wave {v} overwrote ring {ring} row {row} "
+                                    f"with row {have} before block {b} read it")
+                        if have != row:
+                            ok = False
+                if ok:
+                    runnable.append(a)
+            else:
+                runnable.append(a)
+        if not runnable:
+            if all(pc[a] >= len(programs[a]) for a in agents):
+                return None
+            stuck = [(a, programs[a][pc[a]]) for a in agents if pc[a] < len(programs[a])]
+            return f"deadlock: {stuck[:4]}"
+        a = rng.choice(runnable)
+        op = programs[a][pc[a]]
+        if op[0] == "bar":
+            bar_count[a] += 1
+        elif op[0] == "pub":
+            _, ring, row = op
+            slot[(a[0], a[1], ring, row % kring)] = row
+        pc[a] += 1
+
+
+def team_ranges(nz, nteams):
+    for t in range(nteams):
+        yield t * nz // nteams, (t + 1) * nz // nteams, -(-nz // nteams)
+
+
+GRIDS = [(16, 8), (16, 3), (100, 50), (100, 7), (64, 32), (200, 9)]  # (nz, teams)
+
+
+@pytest.mark.parametrize("openb", [False, True])
+@pytest.mark.parametrize("nz,nteams", GRIDS)
+def test_quad_exchange_never_overwrites_or_deadlocks(nz, nteams, openb):
+    rng = random.Random(nz * 131 + nteams * 7 + openb)
+    for j0, jend, L in team_ranges(nz, nteams):
+        prog = quad_program(j0, jend, L, nz, openb)
+        programs = {(b, w): prog for b in range(2) for w in range(W)}
+        for _ in range(6):
+            assert run_team(programs, 2, 8, rng) is None, (nz, nteams, j0, jend)
+
+
+@pytest.mark.parametrize("fd", [False, True])
+@pytest.mark.parametrize("openb", [False, True])
+@pytest.mark.parametrize("nz,nteams", GRIDS)
+def test_pair_exchange_never_overwrites_or_deadlocks(nz, nteams, openb, fd):
+    rng = random.Random(nz * 131 + nteams * 7 + openb * 3 + fd)
+    for nb in (2, 4):
+        for j0, jend, L in team_ranges(nz, nteams):
+            prog = pair_program(j0, jend, L, nz, openb, fd)
+            programs = {(b, w): prog for b in range(nb) for w in range(W)}
+            for _ in range(3):
+                assert run_team(programs, nb, 16, rng) is None, (nb, nz, nteams, j0, jend)
+
+
+def test_model_detects_a_too_shallow_ring():
+    """The model is not vacuous: with a 4-row ring the four-stage schedule
+    lets a block overwrite a slot its partner still needs."""
+    rng = random.Random(5)
+    prog = quad_program(0, 64, 64, 64, False)
+    programs = {(b, w): prog for b in range(2) for w in range(W)}
+    failures = [run_team(programs, 2, 4, rng) for _ in range(20)]
+    assert any(f is not None and "overwrote" in f for f in failures)
+
+
+def test_model_detects_a_wait_that_never_ends():
+    """...and a read of a row no partner ever publishes (beyond the last row
+    its ring receives) is reported as a deadlock."""
+    rng = random.Random(6)
+    prog = quad_program(0, 64, 64, 64, False)
+    bad = [("read", 3, 200) if op == ("read", 3, 0) else op for op in prog]
+    programs = {(b, w): bad for b in range(2) for w in range(W)}
+    f = run_team(programs, 2, 8, rng)
+    assert f is not None and f.startswith("deadlock")

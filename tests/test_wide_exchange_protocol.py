@@ -19,9 +19,15 @@ wavefront (the kernels' loop bounds, ring depths, barrier spacing and read
 conditions), with wavefronts as independent agents that drift between their
 block's barriers, under random interleavings, on grids at the edges of the
 kernels' range (the shortest z ranges, ragged ranges, the open boundary's
-ghost rows), and checks both claims.  A wavefront's reads of its own block's
-LDS rings are covered by the single-block kernels' own tests and are not
-modelled.
+ghost rows), and checks both claims.
+
+The same model checks the block's own LDS moment rings (single-block teams
+included): a row's partials must be written by every wavefront and a
+barrier passed before any wavefront reads them (no wait protects an LDS
+read), and a slot must not be rewritten before every wavefront has read the
+row it holds.  Those are what the barrier spacing (every 4th z step in the
+two-stage kernel, 3 of every 5 in the four-stage one) and the ring depths
+(16 and 8 rows) guarantee.
 """
 import random
 
@@ -30,50 +36,73 @@ import pytest
 W = 3  # wavefronts per block (the kernels have 8; 3 already drift independently)
 
 
-def quad_program(j0, jend, L, nz, openb):
-    """One wavefront's action list in the four-stage kernel (quad_body, NB > 1):
-    ('pub', ring, row), ('read', ring, row), ('bar',)."""
+def quad_program(j0, jend, L, nz, openb, wide=True):
+    """One wavefront's action list in the four-stage kernel (quad_body):
+    ('pub', ring, row) / ('read', ring, row) on the device rings (wide only),
+    ('lpub', ring, row) / ('lread', ring, row) on the block's LDS rings,
+    ('bar',)."""
     ibeg, iend = j0 - 6, j0 + L + 6
 
-    def need(s, r):  # xneed: rows whose totals stage s + 1 uses
+    def need(s, r):  # xneed, and the cells stage s + 1 computes
         m = 6 - 2 * s
         return j0 - m <= r < jend + m and not (openb and (r < 0 or r >= nz))
 
-    prog = [("pub", 0, ibeg), ("pub", 0, ibeg + 1)]
+    prog = [("lpub", 0, ibeg), ("lpub", 0, ibeg + 1)]
+    if wide:
+        prog += [("pub", 0, ibeg), ("pub", 0, ibeg + 1)]
     for i in range(ibeg, iend):
         if (i - ibeg) % 5 in (0, 2, 4):  # barriers at iterations 0, 2, 4 of each group of 5
             prog.append(("bar",))
-        prog.append(("pub", 0, i + 2))
         c = [i, i - 2, i - 4, i - 6]
-        for s in range(4):  # stage s + 1: read ring s at its cell, then publish ring s + 1
-            if need(s, c[s]):
+        # the four stages' partials from LDS, at the top of the z step
+        prog += [("lread", s, c[s]) for s in range(4) if need(s, c[s])]
+        prog.append(("lpub", 0, i + 2))
+        if wide:
+            prog.append(("pub", 0, i + 2))
+        for s in range(4):  # stage s + 1: partner totals of ring s, then its output's partials
+            if wide and need(s, c[s]):
                 prog.append(("read", s, c[s]))
             if s < 3:
-                prog.append(("pub", s + 1, c[s]))
+                if need(s, c[s]):
+                    prog.append(("lpub", s + 1, c[s]))
+                if wide:
+                    prog.append(("pub", s + 1, c[s]))
     return prog
 
 
-def pair_program(j0, jend, L, nz, openb, fd):
-    """One wavefront's action list in the two-stage kernel (pair_body, NB > 1)."""
+def pair_program(j0, jend, L, nz, openb, fd, wide=True):
+    """One wavefront's action list in the two-stage kernel (pair_body)."""
     ibeg, iend = j0 - 5, j0 + L + 4
     last = ibeg + 2 * ((iend - ibeg + 1) // 2) if fd else iend  # FD: pairs of iterations
 
-    def need_a(r):
+    def need_a(r):  # the first stage's cells (and xneed_a)
         return j0 - 4 <= r < jend + 4 and not (openb and (r < 0 or r >= nz))
 
     def need_b(r):
         return j0 <= r < jend
 
-    prog = [("pub", 0, ibeg + 1), ("pub", 0, ibeg + 2), ("pub", 0, ibeg + 3)]
+    prog = [("lpub", 0, ibeg + 1), ("lpub", 0, ibeg + 2), ("lpub", 0, ibeg + 3)]
+    if wide:
+        prog += [("pub", 0, ibeg + 1), ("pub", 0, ibeg + 2), ("pub", 0, ibeg + 3)]
     for i in range(ibeg, last):
-        prog.append(("pub", 0, i + 4))
+        prog.append(("lpub", 0, i + 4))
+        if wide:
+            prog.append(("pub", 0, i + 4))
         if ((i - ibeg) & 3) == 0:  # a barrier every kSync = 4 iterations
             prog.append(("bar",))
-        if need_a(i):
-            prog.append(("read", 0, i))
-        prog.append(("pub", 1, i))
         jb = i - 4
+        # both stages' partials from LDS, right after the barrier
+        if need_a(i):
+            prog.append(("lread", 0, i))
         if need_b(jb):
+            prog.append(("lread", 1, jb))
+        if wide and need_a(i):
+            prog.append(("read", 0, i))
+        if need_a(i):
+            prog.append(("lpub", 1, i))
+        if wide:
+            prog.append(("pub", 1, i))
+        if wide and need_b(jb):
             prog.append(("read", 1, jb))
     return prog
 
@@ -85,6 +114,14 @@ def run_team(programs, nb, kring, rng):
     pc = {a: 0 for a in agents}
     bar_count = {a: 0 for a in agents}        # barriers passed
     slot = {}                                 # (block, wave, ring, row % kring) -> row
+    lds = {}                                  # (block, wave, ring, row % kring) -> (row, barriers)
+    # LDS reads still to come, per (block, ring, row): how many wavefronts will read it
+    pending = {}
+    for a in agents:
+        for op in programs[a]:
+            if op[0] == "lread":
+                key = (a[0], op[1], op[2])
+                pending[key] = pending.get(key, 0) + 1
     while True:
         runnable = []
         for a in agents:
@@ -124,11 +161,30 @@ def run_team(programs, nb, kring, rng):
             return f"deadlock: {stuck[:4]}"
         a = rng.choice(runnable)
         op = programs[a][pc[a]]
+        b, w = a
         if op[0] == "bar":
             bar_count[a] += 1
         elif op[0] == "pub":
             _, ring, row = op
-            slot[(a[0], a[1], ring, row % kring)] = row
+            slot[(b, w, ring, row % kring)] = row
+        elif op[0] == "lpub":
+            _, ring, row = op
+            old = lds.get((b, w, ring, row % kring))
+            if old is not None and pending.get((b, ring, old[0]), 0) > 0:
+                return (f"block {b} wave {w} rewrote LDS ring {ring} row {old[0]} with row {row} "
+                        f"before every wavefront read it")
+            lds[(b, w, ring, row % kring)] = (row, bar_count[a])
+        elif op[0] == "lread":
+            _, ring, row = op
+            for v in range(W):
+                have = lds.get((b, v, ring, row % kring))
+                if have is None or have[0] != row:
+                    return (f"block {b} wave {w} read LDS ring {ring} row {row} holding "
+                            f"{have and have[0]} (wave {v})")
+                if have[1] >= bar_count[a]:
+                    return (f"block {b} wave {w} read LDS ring {ring} row {row} with no barrier "
+                            f"after wave {v} wrote it")
+            pending[(b, ring, row)] -= 1
         pc[a] += 1
 
 
@@ -147,7 +203,7 @@ def test_quad_exchange_never_overwrites_or_deadlocks(nz, nteams, openb):
     for j0, jend, L in team_ranges(nz, nteams):
         prog = quad_program(j0, jend, L, nz, openb)
         programs = {(b, w): prog for b in range(2) for w in range(W)}
-        for _ in range(6):
+        for _ in range(4):
             assert run_team(programs, 2, 8, rng) is None, (nz, nteams, j0, jend)
 
 
@@ -160,8 +216,41 @@ def test_pair_exchange_never_overwrites_or_deadlocks(nz, nteams, openb, fd):
         for j0, jend, L in team_ranges(nz, nteams):
             prog = pair_program(j0, jend, L, nz, openb, fd)
             programs = {(b, w): prog for b in range(nb) for w in range(W)}
-            for _ in range(3):
+            for _ in range(2):
                 assert run_team(programs, nb, 16, rng) is None, (nb, nz, nteams, j0, jend)
+
+
+@pytest.mark.parametrize("openb", [False, True])
+@pytest.mark.parametrize("nz,nteams", [(16, 8), (100, 7), (64, 32), (300, 2)])
+def test_single_block_lds_rings_are_race_free(nz, nteams, openb):
+    """Single-block teams (rows of at most 256 bins): the LDS rings alone."""
+    rng = random.Random(nz * 17 + nteams + openb)
+    for j0, jend, L in team_ranges(nz, nteams):
+        for prog, kring in ((quad_program(j0, jend, L, nz, openb, wide=False), 8),
+                            (pair_program(j0, jend, L, nz, openb, True, wide=False), 16),
+                            (pair_program(j0, jend, L, nz, openb, False, wide=False), 16)):
+            programs = {(0, w): prog for w in range(W)}
+            for _ in range(3):
+                assert run_team(programs, 1, kring, rng) is None, (nz, nteams, j0, jend)
+
+
+def test_model_detects_a_missing_barrier():
+    """The LDS checks are not vacuous: without the barrier at iteration 4 of
+    each group of 5, the four-stage schedule lets a wavefront read a row's
+    partials before a barrier separates them from their writes."""
+    rng = random.Random(7)
+    prog = quad_program(0, 64, 64, 64, False, wide=False)
+    # drop every third barrier (those at iteration 4 of a group)
+    thinned, n = [], 0
+    for op in prog:
+        if op == ("bar",):
+            n += 1
+            if n % 3 == 0:
+                continue
+        thinned.append(op)
+    programs = {(0, w): thinned for w in range(W)}
+    failures = [run_team(programs, 1, 8, rng) for _ in range(10)]
+    assert any(f is not None and "no barrier" in f for f in failures)
 
 
 def test_model_detects_a_too_shallow_ring():

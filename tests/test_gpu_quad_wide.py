@@ -246,3 +246,28 @@ def test_forced_variant_falls_back_on_wide_rows(gpu, monkeypatch, forced, scheme
     pc = C.RunParams(nz=64, nvz=nvz, scheme=scheme, flags=flags, mode=C.FAST)
     with C.Simulation(pc) as sim:
         assert sim.info()["fast_variant"] == expect
+
+
+def test_two_contexts_interleaved_match_solo_runs(gpu):
+    """Two contexts of different wide grids stepped in turns on one device
+    (each with its own exchange rings and ticket counter): every state
+    bit-identical to the same run stepped alone."""
+    po1, pc1 = _quad(256, 512)
+    po2, pc2 = _quad(384, 480, bc=O.OPEN)
+    _, _, s1 = seeded_ic(po1, 0.6, 0.5, 0.9, seed=91, amp=1e-3)
+    _, _, s2 = seeded_ic(po2, 0.7, 0.4, 1.0, seed=92, amp=1e-3)
+    solo = []
+    for pc, s in ((pc1, s1), (pc2, s2)):
+        with C.Simulation(pc) as sim:
+            sim.upload(s)
+            sim.step(25)
+            solo.append(sim.download())
+    print("two wide contexts interleaved", flush=True)
+    with C.Simulation(pc1) as a, C.Simulation(pc2) as b:
+        a.upload(s1)
+        b.upload(s2)
+        for n in (3, 7, 1, 9, 5):
+            a.step(n)
+            b.step(n)
+        outs = [a.download(), b.download()]
+    assert np.array_equal(outs[0], solo[0]) and np.array_equal(outs[1], solo[1])

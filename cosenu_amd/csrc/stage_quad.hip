@@ -81,9 +81,12 @@ template <int NW, bool UNI, bool MAT, bool OPENB, int PHYS = -1, int NB = 1>
 __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, const double ca1,
                                           const double ca2, const double ca3, double *yout, int L,
                                           int j0, int jend, bool active, double (*rings)[kRing][NW][8],
-                                          double *qring, const WideX &X = WideX{})
-// rings: [4][kRing][NW][8] (stage s input moments); qring: the team's NW x kQRing
+                                          double *qring, const WideX &X = WideX{},
+                                          double (*xst)[2][64] = nullptr)
+// rings: [4][kRing][NW][8] (stage s input moments); qring: the team's NW x kQRing;
+// xst (NB > 1): [4][2][64], the partner's partials staged in LDS
 {
+    static_assert(NB == 1 || NB == 2, "teams of one or two blocks");
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wit = wave % NW;
     const int h = lane >> 5;
@@ -173,36 +176,32 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             xput(xslot(rg, X.role, row) + 2 * (wit * 8 + idx), xtag(X.epoch, row), t);
         }
     };
-    // NB > 1: this lane's element of a row's partials summed over the team's
-    // blocks, its own block's from LDS (`own`), partner k's (block role ^ k,
-    // k = 1 .. NB-1) from the words w[k-1] loaded earlier.  The sum is the
-    // tree (own + p1) + (p2 + p3): as IEEE addition commutes, every block of
-    // the team rounds it alike, (P0 + P1) + (P2 + P3) in block numbers.
-    const bool xlane = (lane & 7) < 6; // elements 6, 7 of a wavefront's row are unused
-    auto xload_row = [&](XWords *w, int rg, int row) {
-#pragma unroll
-        for (int k = 1; k < NB; k++)
-            w[k - 1] = xload(xslot(rg, X.role ^ k, row) + 2 * lane);
-    };
-    auto xsum = [&](double own, const XWords *w, int rg, int row) {
-        double p[NB];
-        p[0] = own;
-#pragma unroll
-        for (int k = 1; k < NB; k++)
-            p[k] = xlane ? xvalue(w[k - 1], xslot(rg, X.role ^ k, row) + 2 * lane,
-                                  xtag(X.epoch, row) ^ X.tag_flip, X.ctl, X.wait_ticks, gave_up)
-                         : 0.0;
-        if constexpr (NB == 4)
-            return (p[0] + p[1]) + (p[2] + p[3]);
-        else if constexpr (NB == 2)
-            return p[0] + p[1];
-        else
-            return p[0];
-    };
     // NB > 1: is row r's total needed by stage s + 1 (its cells, see below)
     auto xneed = [&](int s, int r) {
         const int m = 6 - 2 * s;
         return coll && active && r >= j0 - m && r < jend + m && !(OPENB && (r < 0 || r >= nz));
+    };
+    // NB > 1: the partner block's partials of the rows the stages take next
+    // pass through an LDS staging buffer, xst[ring][row & 1][64 elements].
+    // Wavefront r < 4 loads ring r's partner words of the row its stage needs
+    // at the next z step, checks their tags at the end of this step (waiting,
+    // bounded, for words not yet written) and writes the values to the
+    // buffer.  After the next barrier (one on every z step when NB > 1) every
+    // wavefront adds them to its own block's partials from the LDS rings:
+    // own + partner, which both blocks of the team round alike (IEEE
+    // addition commutes).  One wavefront per ring loads a row instead of
+    // eight, and holds one ring's words instead of four (FD, nz = 16384,
+    // nvz = 512: 0.444 against 0.474 ms per step; nvz = 480, open: 0.619
+    // against 0.781).
+    const bool xlane = (lane & 7) < 6; // elements 6, 7 of a wavefront's row are unused
+    const int xr = wit;                // the ring this wavefront stages (< 4)
+    XWords xw = {0ull, 0ull};
+    auto xstage = [&](int row) {
+        if (xr < 4 && xneed(xr, row) && xlane) {
+            const double v = xvalue(xw, xslot(xr, X.role ^ 1, row) + 2 * lane,
+                                    xtag(X.epoch, row) ^ X.tag_flip, X.ctl, X.wait_ticks, gave_up);
+            xst[xr][row & 1][lane] = v;
+        }
     };
     // right-hand side at the window centre of z row r: mx = this lane's
     // moment_partial of the row's ring slot, hmr = Hm[r] (MAT); the terms in
@@ -282,11 +281,15 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         xpub(0, ibeg, t0);
         xpub(0, ibeg + 1, t1);
     }
-    // NB > 1: the partners' words of the rows the four stages take next
-    // (ring s, row i - 2 s: published 2 iterations before it is needed),
-    // loaded one iteration ahead, right after the previous row's words were
-    // used
-    XRingRows<4, NB> xw; // (empty for NB = 1)
+    // NB > 1: the rows the stages take at the first z step, staged before its
+    // barrier
+    if constexpr (NB > 1) {
+        if (xr < 4) {
+            const int row = ibeg - 2 * xr;
+            xw = xload(xslot(xr, X.role ^ 1, row) + 2 * lane);
+            xstage(row);
+        }
+    }
     double nxt[4];
     load4(A.yin, ibeg + 3, nxt); // the row that enters the window after iteration ibeg
     // The prologue's loads have landed before the loop (the wait-count
@@ -300,7 +303,7 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
     auto step = [&](const int i, auto tc) __attribute__((always_inline)) {
         constexpr int T = decltype(tc)::value;
         constexpr int old = T % 5, cen0 = (T + 2) % 5, top0 = (T + 4) % 5, cen1 = (T + 3) % 5;
-        if (T % 2 == 0) // the same iterations in every team
+        if (NB > 1 || T % 2 == 0) // the same iterations in every team
             __syncthreads();
         const int c1 = i, c2 = i - 2, c3 = i - 4, c4 = i - 6;
         // the stages' input moments (rows stored 2 iterations ago, behind a
@@ -312,23 +315,18 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             mx[2] = moment_partial<NW>(rings[2][c3 & (kRing - 1)], lane);
             mx[3] = moment_partial<NW>(rings[3][c4 & (kRing - 1)], lane);
         }
-        // NB > 1: stage r + 1's totals at its cell c, summed right before
-        // the stage (the partners published row c at stage r of their
-        // previous iteration, or, r = 0, at its start); after the stage, the
-        // words of its next cell are loaded (~1.5 iterations after the
-        // partners published them)
-        auto xsum_at = [&](auto rc, int c) __attribute__((always_inline)) {
-            constexpr int r = decltype(rc)::value;
-            if constexpr (NB > 1) {
-                if (xneed(r, c))
-                    mx[r] = xsum(mx[r], xw.r[r], r, c);
-            }
-        };
-        auto xnext = [&](auto rc, int c) __attribute__((always_inline)) {
-            constexpr int r = decltype(rc)::value;
-            if constexpr (NB > 1)
-                xload_row(xw.r[r], r, c + 1);
-        };
+        // NB > 1: the partner's partials of the four stages' rows, staged
+        // behind the barrier; the staging wavefront's load for the next z
+        // step, issued before this step's window load
+        if constexpr (NB > 1) {
+            const int cc[4] = {c1, c2, c3, c4};
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (xneed(r, cc[r]))
+                    mx[r] = mx[r] + xst[r][cc[r] & 1][lane];
+            if (xr < 4) // issued before the window load
+                xw = xload(xslot(xr, X.role ^ 1, i + 1 - 2 * xr) + 2 * lane);
+        }
         // the matter potential of the four stages' rows, loaded ahead of use
         double hmv[4] = {0.0, 0.0, 0.0, 0.0};
         if (MAT && mat) {
@@ -363,7 +361,6 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
             }
         };
         // stage 1 at c1: Y2 = y + (dt/2) K1
-        xsum_at(Rot<0>{}, c1);
         double o[4] = {0.0, 0.0, 0.0, 0.0};
         double tp = 0.0; // the published total of the stage's output row
         if (c1 >= j0 - 6 && c1 < jend + 6 && !(OPENB && (c1 < 0 || c1 >= nz))) {
@@ -376,7 +373,6 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 tp = moments(1, c1, o[0], o[1], o[2], o[3]);
         }
         xpub(1, c1, tp);
-        xnext(Rot<0>{}, c1);
         if (OPENB) {
             ghost(w1, c1, o);
         } else {
@@ -386,7 +382,6 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
         }
         // stage 2 at c2: Y3; Q and y to the rings.  y of this cell is the y
         // window's oldest row (shifted out at the end of the iteration).
-        xsum_at(Rot<1>{}, c2);
         tp = 0.0;
         if (c2 >= j0 - 4 && c2 < jend + 4 && !(OPENB && (c2 < 0 || c2 >= nz))) {
             double K[4];
@@ -406,7 +401,6 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 o[f] = 0.0;
         }
         xpub(2, c2, tp);
-        xnext(Rot<1>{}, c2);
         if (OPENB) {
             ghost(w2, c2, o);
         } else {
@@ -415,7 +409,6 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 w2[f][old] = o[f];
         }
         // stage 3 at c3: Y4 = y + dt K3; T = y + (Q + (Y4 - y))/3 replaces Q
-        xsum_at(Rot<2>{}, c3);
         tp = 0.0;
         if (c3 >= j0 - 2 && c3 < jend + 2 && !(OPENB && (c3 < 0 || c3 >= nz))) {
             double K[4];
@@ -435,7 +428,6 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 o[f] = 0.0;
         }
         xpub(3, c3, tp);
-        xnext(Rot<2>{}, c3);
         if (OPENB) {
             ghost(w3, c3, o);
         } else {
@@ -444,7 +436,6 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                 w3[f][old] = o[f];
         }
         // stage 4 at c4: y' = T + (dt/6) K4
-        xsum_at(Rot<3>{}, c4);
         if (c4 >= j0 && c4 < jend) {
             double K[4];
             rhs(w3, Rot<T + 1>{}, mx[3], hmv[3], K);
@@ -455,7 +446,10 @@ __device__ __forceinline__ void quad_body(const StageArgs &A, const double ca0, 
                     out[(size_t)f * VP] = fma(ca3, K[f], tslot(c4, f));
             }
         }
-        xnext(Rot<3>{}, c4);
+        if constexpr (NB > 1) { // (checked here: the load had the whole z step to land)
+            if (xr < 4)
+                xstage(i + 1 - 2 * xr);
+        }
         // the y window advances (OPENB: row i+3 above the grid extrapolates
         // from rows i+2 and i+1)
         const bool ex = OPENB && i + 3 >= nz;
@@ -510,11 +504,13 @@ void cosenu_quad_stage(StageArgs A, double ca0, double ca1, double ca2, double c
 
 // Rows of 512 bins: a team of NB = 2 blocks of 8 wavefronts per contiguous z
 // range (range mode only: the grid is nteams * NB blocks, every team the same
-// trip count).  Teams form by ticket (WideX).  (Teams of 4 blocks, for rows
-// of 1024 bins, would hold 3 partners' words of 4 rings, 48 more VGPRs: they
-// spill 192-300 B per lane and ran 1.9 ms per step against 1.05 ms for the
-// two-stage kernel at nz = 8192, so those rows take the two-stage kernel.)
-template <bool UNI, bool MAT, bool OPENB, int NB>
+// trip count).  Teams form by ticket (WideX); the partner's partials come in
+// through xst, one loader wavefront per ring.  (Teams of 4 blocks, for rows
+// of 1024 bins, in a form that held 3 partners' words of 4 rings in every
+// wavefront's registers, spilled 192-300 B per lane and ran 1.9 ms per step
+// against 1.05 ms for the two-stage kernel at nz = 8192: those rows take the
+// two-stage kernel.)
+template <bool UNI, bool MAT, bool OPENB, int NB, int PHYS = -1>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
 void cosenu_quad_wide(StageArgs A, double ca0, double ca1, double ca2, double ca3, double *yout,
                       int nteams, unsigned long long *slots, unsigned long long *ctl,
@@ -523,13 +519,14 @@ void cosenu_quad_wide(StageArgs A, double ca0, double ca1, double ca2, double ca
     constexpr int NW = 8;
     __shared__ double rings[4][kRing][NW][8];
     __shared__ double qring[NW * kQRing];
+    __shared__ double xst[4][2][64];
     const WideX X = wide_join<NB>(slots, ctl, wait_ticks, tag_flip, nteams);
     const bool active = X.team < nteams;
     const int L = (A.nz + nteams - 1) / nteams;
     const int j0 = active ? (int)(((long)X.team * A.nz) / nteams) : 0;
     const int j1 = active ? (int)(((long)(X.team + 1) * A.nz) / nteams) : 0;
-    quad_body<NW, UNI, MAT, OPENB, -1, NB>(A, ca0, ca1, ca2, ca3, yout, L, j0, j1, active, rings,
-                                           qring, X);
+    quad_body<NW, UNI, MAT, OPENB, PHYS, NB>(A, ca0, ca1, ca2, ca3, yout, L, j0, j1, active, rings,
+                                           qring, X, xst);
 }
 
 template <int NB>
@@ -545,8 +542,10 @@ hipError_t launch_wide_nb(const StageArgs &a, const double ca[4], double *yout, 
     else if (a.bc == 1)
         hipLaunchKernelGGL((cosenu_quad_wide<false, false, true, NB>), grid, block, 0, s, a, ca[0],
                            ca[1], ca[2], ca[3], yout, nteams, slots, ctl, ticks, flip);
-    else if (uni) // (the benchmark physics fixed at compile time spills here: 0.508 against
-                  // 0.467 ms per step at nz = 16384, nvz = 512)
+    else if (uni && a.flags == kStdPhys) // the benchmark physics fixed at compile time
+        hipLaunchKernelGGL((cosenu_quad_wide<true, false, false, NB, kStdPhys>), grid, block, 0, s,
+                           a, ca[0], ca[1], ca[2], ca[3], yout, nteams, slots, ctl, ticks, flip);
+    else if (uni)
         hipLaunchKernelGGL((cosenu_quad_wide<true, true, false, NB>), grid, block, 0, s, a, ca[0],
                            ca[1], ca[2], ca[3], yout, nteams, slots, ctl, ticks, flip);
     else

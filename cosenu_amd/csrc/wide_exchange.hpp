@@ -79,15 +79,6 @@ __device__ inline WideX wide_join(unsigned long long *slots, unsigned long long 
     return X;
 }
 
-// the partner words of the next rows of NR moment rings
-template <int NR, int NB>
-struct XRingRows {
-    XWords r[NR][NB - 1] = {};
-};
-template <int NR>
-struct XRingRows<NR, 1> {
-};
-
 __device__ inline unsigned xtag(unsigned epoch, int row)
 {
     return (epoch << 24) | ((unsigned)(row + 64) & 0xFFFFFFu);

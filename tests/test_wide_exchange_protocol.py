@@ -4,10 +4,13 @@ Rows of 512 and 1024 bins split a z range's team over several blocks
 (cosenu_pair_wide in stage_pair.hip, cosenu_quad_wide in stage_quad.hip).
 Every wavefront publishes its partial moments of a row into a device-memory
 ring of kRing rows (slot = row mod kRing, tagged with the row), on every z
-step whatever the row, and the partner blocks' wavefronts read a row's slots
-when a stage needs that row's totals, waiting until the slot carries the
-row's tag.  Nothing but that data dependency (and each block's own barriers)
-orders the blocks, so the design rests on two claims:
+step whatever the row.  In the two-stage kernel the partner blocks'
+wavefronts read a row's slots when a stage needs that row's totals; in the
+four-stage kernel one staging wavefront per ring reads them a z step ahead
+and passes the values to its block through an LDS buffer of two rows.  A
+reader waits until the slot carries the row's tag.  Nothing but that data
+dependency (and each block's own barriers) orders the blocks, so the design
+rests on two claims:
 
 * no slot is overwritten with a later row before every partner wavefront
   that needs the earlier row has read it (a reader would otherwise wait for
@@ -22,24 +25,27 @@ kernels' range (the shortest z ranges, ragged ranges, the open boundary's
 ghost rows), and checks both claims.
 
 The same model checks the block's own LDS moment rings (single-block teams
-included): a row's partials must be written by every wavefront and a
+included) and the staging buffer: a row's values must be written and a
 barrier passed before any wavefront reads them (no wait protects an LDS
 read), and a slot must not be rewritten before every wavefront has read the
 row it holds.  Those are what the barrier spacing (every 4th z step in the
-two-stage kernel, 3 of every 5 in the four-stage one) and the ring depths
-(16 and 8 rows) guarantee.
+two-stage kernel; 3 of every 5 in the single-block four-stage one, every z
+step in the two-block one) and the ring depths (16 and 8 rows) guarantee.
 """
 import random
 
 import pytest
 
-W = 3  # wavefronts per block (the kernels have 8; 3 already drift independently)
+W = 3  # wavefronts per block in the two-stage model (the kernels have 8)
+WQ = 4  # in the four-stage model: one staging wavefront per ring
 
 
-def quad_program(j0, jend, L, nz, openb, wide=True):
-    """One wavefront's action list in the four-stage kernel (quad_body):
+def quad_program(j0, jend, L, nz, openb, wide=True, wave=0):
+    """Wavefront `wave`'s action list in the four-stage kernel (quad_body):
     ('pub', ring, row) / ('read', ring, row) on the device rings (wide only),
     ('lpub', ring, row) / ('lread', ring, row) on the block's LDS rings,
+    ('spub', ring, row) / ('sread', ring, row) on the LDS staging buffer of
+    the partner's partials (wide only; wavefront r < 4 stages ring r),
     ('bar',)."""
     ibeg, iend = j0 - 6, j0 + L + 6
 
@@ -47,26 +53,35 @@ def quad_program(j0, jend, L, nz, openb, wide=True):
         m = 6 - 2 * s
         return j0 - m <= r < jend + m and not (openb and (r < 0 or r >= nz))
 
+    def stage(row):  # the staging wavefront: check the partner's words, write the buffer
+        if wide and wave < 4 and need(wave, row):
+            return [("read", wave, row), ("spub", wave, row)]
+        return []
+
     prog = [("lpub", 0, ibeg), ("lpub", 0, ibeg + 1)]
     if wide:
         prog += [("pub", 0, ibeg), ("pub", 0, ibeg + 1)]
+    prog += stage(ibeg - 2 * wave)
     for i in range(ibeg, iend):
-        if (i - ibeg) % 5 in (0, 2, 4):  # barriers at iterations 0, 2, 4 of each group of 5
+        # barriers at iterations 0, 2, 4 of each group of 5; on every
+        # iteration with a staging buffer
+        if wide or (i - ibeg) % 5 in (0, 2, 4):
             prog.append(("bar",))
         c = [i, i - 2, i - 4, i - 6]
-        # the four stages' partials from LDS, at the top of the z step
+        # the four stages' partials from LDS (and the partner's, staged), at
+        # the top of the z step
         prog += [("lread", s, c[s]) for s in range(4) if need(s, c[s])]
+        if wide:
+            prog += [("sread", s, c[s]) for s in range(4) if need(s, c[s])]
         prog.append(("lpub", 0, i + 2))
         if wide:
             prog.append(("pub", 0, i + 2))
-        for s in range(4):  # stage s + 1: partner totals of ring s, then its output's partials
-            if wide and need(s, c[s]):
-                prog.append(("read", s, c[s]))
-            if s < 3:
-                if need(s, c[s]):
-                    prog.append(("lpub", s + 1, c[s]))
-                if wide:
-                    prog.append(("pub", s + 1, c[s]))
+        for s in range(3):  # stages 1-3: their outputs' partials
+            if need(s, c[s]):
+                prog.append(("lpub", s + 1, c[s]))
+            if wide:
+                prog.append(("pub", s + 1, c[s]))
+        prog += stage(i + 1 - 2 * wave)  # after stage 4: the next z step's row
     return prog
 
 
@@ -107,20 +122,25 @@ def pair_program(j0, jend, L, nz, openb, fd, wide=True):
     return prog
 
 
-def run_team(programs, nb, kring, rng):
-    """Interleave nb blocks x W wavefronts at random.  Returns None when every
-    agent finished, else a description of the failure."""
-    agents = [(b, w) for b in range(nb) for w in range(W)]
+def run_team(programs, nb, kring, rng, dev_kring=None):
+    """Interleave the agents (block, wavefront) of `programs` at random, with
+    LDS rings of kring rows and device rings of dev_kring rows (default:
+    kring).  Returns None when every agent finished, else a description of
+    the failure."""
+    dev_kring = dev_kring or kring
+    agents = sorted(programs)
+    waves = {b: [w for (bb, w) in agents if bb == b] for b in range(nb)}
     pc = {a: 0 for a in agents}
     bar_count = {a: 0 for a in agents}        # barriers passed
     slot = {}                                 # (block, wave, ring, row % kring) -> row
     lds = {}                                  # (block, wave, ring, row % kring) -> (row, barriers)
-    # LDS reads still to come, per (block, ring, row): how many wavefronts will read it
+    stg = {}                                  # (block, ring, row % 2) -> (row, barriers)
+    # LDS reads still to come, per (kind, block, ring, row): how many wavefronts will read it
     pending = {}
     for a in agents:
         for op in programs[a]:
-            if op[0] == "lread":
-                key = (a[0], op[1], op[2])
+            if op[0] in ("lread", "sread"):
+                key = (op[0], a[0], op[1], op[2])
                 pending[key] = pending.get(key, 0) + 1
     while True:
         runnable = []
@@ -135,7 +155,7 @@ def run_team(programs, nb, kring, rng):
                 if all(bar_count[(b, v)] > bar_count[a] or
                        (pc[(b, v)] < len(programs[(b, v)]) and
                         programs[(b, v)][pc[(b, v)]][0] == "bar" and
-                        bar_count[(b, v)] == bar_count[a]) for v in range(W)):
+                        bar_count[(b, v)] == bar_count[a]) for v in waves[b]):
                     runnable.append(a)
             elif op[0] == "read":
                 _, ring, row = op
@@ -143,8 +163,8 @@ def run_team(programs, nb, kring, rng):
                 for pb in range(nb):
                     if pb == b:
                         continue
-                    for v in range(W):
-                        have = slot.get((pb, v, ring, row % kring))
+                    for v in waves[pb]:
+                        have = slot.get((pb, v, ring, row % dev_kring))
                         if have is not None and have > row:
                             return (f"block {pb} wave {v} overwrote ring {ring} row {row} "
                                     f"with row {have} before block {b} read it")
@@ -166,17 +186,33 @@ def run_team(programs, nb, kring, rng):
             bar_count[a] += 1
         elif op[0] == "pub":
             _, ring, row = op
-            slot[(b, w, ring, row % kring)] = row
+            slot[(b, w, ring, row % dev_kring)] = row
         elif op[0] == "lpub":
             _, ring, row = op
             old = lds.get((b, w, ring, row % kring))
-            if old is not None and pending.get((b, ring, old[0]), 0) > 0:
+            if old is not None and pending.get(("lread", b, ring, old[0]), 0) > 0:
                 return (f"block {b} wave {w} rewrote LDS ring {ring} row {old[0]} with row {row} "
                         f"before every wavefront read it")
             lds[(b, w, ring, row % kring)] = (row, bar_count[a])
+        elif op[0] == "spub":
+            _, ring, row = op
+            old = stg.get((b, ring, row % 2))
+            if old is not None and pending.get(("sread", b, ring, old[0]), 0) > 0:
+                return (f"block {b} wave {w} rewrote staged ring {ring} row {old[0]} with row "
+                        f"{row} before every wavefront read it")
+            stg[(b, ring, row % 2)] = (row, bar_count[a])
+        elif op[0] == "sread":
+            _, ring, row = op
+            have = stg.get((b, ring, row % 2))
+            if have is None or have[0] != row:
+                return f"block {b} wave {w} read staged ring {ring} row {row} holding {have}"
+            if have[1] >= bar_count[a]:
+                return (f"block {b} wave {w} read staged ring {ring} row {row} with no barrier "
+                        f"after it was written")
+            pending[("sread", b, ring, row)] -= 1
         elif op[0] == "lread":
             _, ring, row = op
-            for v in range(W):
+            for v in waves[b]:
                 have = lds.get((b, v, ring, row % kring))
                 if have is None or have[0] != row:
                     return (f"block {b} wave {w} read LDS ring {ring} row {row} holding "
@@ -184,7 +220,7 @@ def run_team(programs, nb, kring, rng):
                 if have[1] >= bar_count[a]:
                     return (f"block {b} wave {w} read LDS ring {ring} row {row} with no barrier "
                             f"after wave {v} wrote it")
-            pending[(b, ring, row)] -= 1
+            pending[("lread", b, ring, row)] -= 1
         pc[a] += 1
 
 
@@ -201,8 +237,8 @@ GRIDS = [(16, 8), (16, 3), (100, 50), (100, 7), (64, 32), (200, 9)]  # (nz, team
 def test_quad_exchange_never_overwrites_or_deadlocks(nz, nteams, openb):
     rng = random.Random(nz * 131 + nteams * 7 + openb)
     for j0, jend, L in team_ranges(nz, nteams):
-        prog = quad_program(j0, jend, L, nz, openb)
-        programs = {(b, w): prog for b in range(2) for w in range(W)}
+        programs = {(b, w): quad_program(j0, jend, L, nz, openb, wave=w)
+                    for b in range(2) for w in range(WQ)}
         for _ in range(4):
             assert run_team(programs, 2, 8, rng) is None, (nz, nteams, j0, jend)
 
@@ -253,13 +289,20 @@ def test_model_detects_a_missing_barrier():
     assert any(f is not None and "no barrier" in f for f in failures)
 
 
-def test_model_detects_a_too_shallow_ring():
-    """The model is not vacuous: with a 4-row ring the four-stage schedule
-    lets a block overwrite a slot its partner still needs."""
+@pytest.mark.parametrize("kind", ["quad", "pair"])
+def test_model_detects_a_too_shallow_device_ring(kind):
+    """The model is not vacuous: with 2-row device rings (the LDS rings as
+    built) a block overwrites a slot its partner still needs."""
     rng = random.Random(5)
-    prog = quad_program(0, 64, 64, 64, False)
-    programs = {(b, w): prog for b in range(2) for w in range(W)}
-    failures = [run_team(programs, 2, 4, rng) for _ in range(20)]
+    if kind == "quad":
+        programs = {(b, w): quad_program(0, 64, 64, 64, False, wave=w) for b in range(2)
+                    for w in range(WQ)}
+        kring = 8
+    else:
+        prog = pair_program(0, 64, 64, 64, False, False)
+        programs = {(b, w): prog for b in range(2) for w in range(W)}
+        kring = 16
+    failures = [run_team(programs, 2, kring, rng, dev_kring=2) for _ in range(20)]
     assert any(f is not None and "overwrote" in f for f in failures)
 
 
@@ -267,8 +310,11 @@ def test_model_detects_a_wait_that_never_ends():
     """...and a read of a row no partner ever publishes (beyond the last row
     its ring receives) is reported as a deadlock."""
     rng = random.Random(6)
-    prog = quad_program(0, 64, 64, 64, False)
-    bad = [("read", 3, 200) if op == ("read", 3, 0) else op for op in prog]
-    programs = {(b, w): bad for b in range(2) for w in range(W)}
+    programs = {}
+    for b in range(2):
+        for w in range(WQ):
+            prog = quad_program(0, 64, 64, 64, False, wave=w)
+            programs[(b, w)] = [("read", op[1], 200) if op[0] == "read" and op[2] == 5 else op
+                                for op in prog]
     f = run_team(programs, 2, 8, rng)
     assert f is not None and f.startswith("deadlock")

@@ -91,8 +91,10 @@ template <int NW, int SCHEME, int PAIR, bool UNI, bool OPENB, int NB = 1, int PH
 __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &A2, int L, int j0,
                                           int jend, bool active, double (*ra)[NW][8],
                                           double (*rb)[NW][8], double *yring,
-                                          const WideX &X = WideX{})
-// ra, rb: [kRing][NW][8]; yring: the team's NW x kYRing doubles (stages 3+4)
+                                          const WideX &X = WideX{}, double *xst = nullptr)
+// ra, rb: [kRing][NW][8]; yring: the team's NW x kYRing doubles (stages 3+4);
+// xst (NB > 1): [2 rings][NB - 1 partners][2 rows][64 elements], the partner
+// blocks' partials staged in LDS
 {
     // The right-hand side's terms and the RK4 combinations below write their
     // fused multiply-adds out and nothing else is fused here (the WENO7 code
@@ -187,25 +189,33 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             xput(xslot(rg, X.role, row) + 2 * (wit * 8 + idx), xtag(X.epoch, row), t);
         }
     };
-    // NB > 1: this lane's element of a row's partials summed over the team's
-    // blocks, its own block's from LDS (`own`), partner k's (block role ^ k,
-    // k = 1 .. NB-1) from the words w[k-1] loaded earlier.  The sum is the
-    // tree (own + p1) + (p2 + p3): as IEEE addition commutes, every block of
-    // the team rounds it alike, (P0 + P1) + (P2 + P3) in block numbers.
+    // NB > 1: the partner blocks' partials come in through LDS.  Wavefront
+    // wit < 2 (NB - 1) stages ring lrg = wit / (NB - 1) from partner k =
+    // wit % (NB - 1) + 1 (block role ^ k): it loads the words of the ring's
+    // next row in the middle of a z step, checks their tags at its end and
+    // writes the value to xst; the barrier that opens every z step orders
+    // that write before the other wavefronts' reads.  The other wavefronts
+    // issue no exchange loads, and no wavefront holds partner words in
+    // registers across a step.
+    constexpr int NX = NB > 1 ? NB - 1 : 1;
+    const int lrg = wit / NX, lk = wit % NX + 1;
+    const bool xld = NB > 1 && wit < 2 * NX;
     const bool xlane = (lane & 7) < 6; // elements 6, 7 of a wavefront's row are unused
-    auto xload_row = [&](XWords *w, int rg, int row) {
-#pragma unroll
-        for (int k = 1; k < NB; k++)
-            w[k - 1] = xload(xslot(rg, X.role ^ k, row) + 2 * lane);
+    XWords xw = {0ull, 0ull};
+    auto xst_at = [&](int rg, int k, int row) -> double & {
+        return xst[((rg * NX + k - 1) * 2 + (row & 1)) * 64 + lane];
     };
-    auto xsum = [&](double own, const XWords *w, int rg, int row) {
+    // this lane's element of a row's partials summed over the team's blocks,
+    // its own block's from LDS (`own`), partner k's (k = 1 .. NB-1) staged.
+    // The sum is the tree (own + p1) + (p2 + p3): as IEEE addition commutes,
+    // every block of the team rounds it alike, (P0 + P1) + (P2 + P3) in
+    // block numbers.
+    auto xsum = [&](double own, int rg, int row) {
         double p[NB];
         p[0] = own;
 #pragma unroll
         for (int k = 1; k < NB; k++)
-            p[k] = xlane ? xvalue(w[k - 1], xslot(rg, X.role ^ k, row) + 2 * lane,
-                                  xtag(X.epoch, row) ^ X.tag_flip, X.ctl, X.wait_ticks, gave_up)
-                         : 0.0;
+            p[k] = xlane ? xst_at(rg, k, row) : 0.0;
         if constexpr (NB == 4)
             return (p[0] + p[1]) + (p[2] + p[3]);
         else if constexpr (NB == 2)
@@ -319,13 +329,6 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
     double Fa2[4] = {0.0, 0.0, 0.0, 0.0}, Fb2[4] = {0.0, 0.0, 0.0, 0.0}; // OPENB FV: faces before last
     double wprev[4] = {0.0, 0.0, 0.0, 0.0}; // the row before the window
     double yn[4] = {0.0, 0.0, 0.0, 0.0};    // stages 3+4: y of the next first-stage cell
-    // NB > 1: the other blocks' partials of the rows the two stages take next
-    // (first stage: row i, published by the partners 4 iterations before it
-    // is needed; second stage: row jb, published when their first stage
-    // computed it), loaded one iteration ahead: the first stage's right
-    // after its row's words were used, the second's at the end of the
-    // iteration
-    XRows<NB> xr; // (empty for NB = 1)
 
     auto iteration = [&](const int i) {
         // NB > 1: is row r's total needed by the first (a) or second (b) stage
@@ -333,6 +336,11 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
             return coll && active && r >= j0 - 4 && r < jend + 4 && !(OPENB && (r < 0 || r >= nz));
         };
         auto xneed_b = [&](int r) { return coll && active && r >= j0 && r < jend; };
+        // NB > 1: the row a loader wavefront stages during this iteration,
+        // for the next: the first stage's row i + 1 (published by the
+        // partner 3 iterations before), the second stage's row i - 3
+        // (published by the partner's first stage 3 iterations before)
+        const int xrow = lrg == 0 ? i + 1 : i - 3;
         double nxt[4];
         auto window_load = [&] {
             if (PF) {
@@ -389,7 +397,8 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         if (SCHEME == 0 && advect)
             face(wa, F);
 
-        if (((i - ibeg) & (kSync - 1)) == 0) // the same iterations in every team
+        // the same iterations in every team; every iteration with staging
+        if (NB > 1 || ((i - ibeg) & (kSync - 1)) == 0)
             __syncthreads();
         // the two stages' input moments (rows stored 4 iterations ago, behind
         // a barrier), read before the first stage needs them
@@ -397,8 +406,9 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
         double mxb = coll ? moment_partial<NW>(rb[jb & (kRing - 1)], lane) : 0.0;
         if constexpr (NB > 1) {
             if (xa)
-                mxa = xsum(mxa, xr.a, 0, i);
-            xload_row(xr.a, 0, i + 1);
+                mxa = xsum(mxa, 0, i);
+            if (xld) // issued before the window load
+                xw = xload(xslot(lrg, X.role ^ lk, xrow) + 2 * lane);
         }
 
         // first stage at cell i: its output becomes the newest row of wb
@@ -450,7 +460,7 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
                 face(wb, G);
             if constexpr (NB > 1) {
                 if (xb)
-                    mxb = xsum(mxb, xr.b, 1, jb);
+                    mxb = xsum(mxb, 1, jb);
             }
             if (cell_b) {
                 double K[4], Gl[4], Gr[4];
@@ -492,8 +502,13 @@ __device__ __forceinline__ void pair_body(const StageArgs &A1, const StageArgs &
                 Fb[f] = G[f];
             }
         }
-        if constexpr (NB > 1)
-            xload_row(xr.b, 1, jb + 1);
+        if constexpr (NB > 1) { // (checked here: the load had half a z step to land)
+            const int r = xrow;
+            if (xld && (lrg == 0 ? xneed_a(r) : xneed_b(r)) && xlane)
+                xst_at(lrg, lk, r) = xvalue(xw, xslot(lrg, X.role ^ lk, r) + 2 * lane,
+                                            xtag(X.epoch, r) ^ X.tag_flip, X.ctl, X.wait_ticks,
+                                            gave_up);
+        }
         // OPENB: row i+5 above the grid extrapolates from rows i+4, i+3, one
         // row after another as the reference fills its ghost cells
         const bool extrap = OPENB && i + 5 >= nz;
@@ -589,13 +604,14 @@ void cosenu_pair_wide(StageArgs A1, StageArgs A2, int nteams, unsigned long long
     __shared__ double ring_a[kRing][NW][8];
     __shared__ double ring_b[kRing][NW][8];
     __shared__ double yring[PAIR == 1 ? NW * kYRing : 1];
+    __shared__ double xst[2 * (NB - 1) * 2 * 64];
     const WideX X = wide_join<NB>(slots, ctl, wait_ticks, tag_flip, nteams);
     const bool active = X.team < nteams;
     const int L = (A1.nz + nteams - 1) / nteams;
     const int j0 = active ? (int)(((long)X.team * A1.nz) / nteams) : 0;
     const int j1 = active ? (int)(((long)(X.team + 1) * A1.nz) / nteams) : 0;
     pair_body<NW, SCHEME, PAIR, UNI, OPENB, NB, PHYS>(A1, A2, L, j0, j1, active, ring_a, ring_b,
-                                                      yring, X);
+                                                      yring, X, xst);
 }
 
 template <int SCHEME, int NB>

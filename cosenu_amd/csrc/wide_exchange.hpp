@@ -11,10 +11,11 @@ namespace fastk {
 // wavefronts owns 256 bins of the team's z range, and the moments
 // of a z row are summed over the team's blocks through device memory.  Every
 // wavefront publishes its 6 partials of a row (the values it writes to its
-// block's LDS ring) to a ring in device memory as well, and the partner
-// blocks read them when they need the row's totals (4 z steps later in the
-// two-stage kernel, 2 in the four-stage one), having loaded them one z step
-// ahead.
+// block's LDS ring) to a ring in device memory as well.  In each partner
+// block one loader wavefront per ring (and partner) loads a row's words
+// about a z step before its block needs the row's totals, checks their tags
+// and stages the values in LDS, where every wavefront of the block reads
+// them after the next barrier.
 //
 // A value travels as two 64-bit words, each holding 32 bits of the double
 // and a 32-bit tag (the launch epoch and the z row): a word is written and
@@ -43,15 +44,6 @@ struct WideX {
 
 struct XWords {
     unsigned long long w0, w1;
-};
-
-// the partner words of the next rows of the two stages' moment rings
-template <int NB>
-struct XRows {
-    XWords a[NB - 1] = {}, b[NB - 1] = {};
-};
-template <>
-struct XRows<1> {
 };
 
 // A block of a multi-block launch joins its team: a ticket from the launch's

@@ -4,10 +4,10 @@ Rows of 512 and 1024 bins split a z range's team over several blocks
 (cosenu_pair_wide in stage_pair.hip, cosenu_quad_wide in stage_quad.hip).
 Every wavefront publishes its partial moments of a row into a device-memory
 ring of kRing rows (slot = row mod kRing, tagged with the row), on every z
-step whatever the row.  In the two-stage kernel the partner blocks'
-wavefronts read a row's slots when a stage needs that row's totals; in the
-four-stage kernel one staging wavefront per ring reads them a z step ahead
-and passes the values to its block through an LDS buffer of two rows.  A
+step whatever the row.  In each partner block one loader wavefront per ring
+(and, in teams of 4 blocks, per partner) reads a row's slots about a z step
+before its block needs the row's totals and passes the values to the block
+through an LDS staging buffer of two rows.  A
 reader waits until the slot carries the row's tag.  Nothing but that data
 dependency (and each block's own barriers) orders the blocks, so the design
 rests on two claims:
@@ -29,23 +29,30 @@ included) and the staging buffer: a row's values must be written and a
 barrier passed before any wavefront reads them (no wait protects an LDS
 read), and a slot must not be rewritten before every wavefront has read the
 row it holds.  Those are what the barrier spacing (every 4th z step in the
-two-stage kernel; 3 of every 5 in the single-block four-stage one, every z
-step in the two-block one) and the ring depths (16 and 8 rows) guarantee.
+single-block two-stage kernel, 3 of every 5 in the single-block four-stage
+one, every z step in the multi-block ones) and the ring depths (16 and 8 rows) guarantee.
 """
 import random
 
 import pytest
 
-W = 3  # wavefronts per block in the two-stage model (the kernels have 8)
+W = 3  # wavefronts per block in the single-block models (the kernels have 8)
 WQ = 4  # in the four-stage model: one staging wavefront per ring
+
+
+def pair_waves(nb):
+    """Wavefronts per block in the multi-block two-stage model: a loader per
+    ring and partner, plus one that only consumes."""
+    return 2 * (nb - 1) + 1
 
 
 def quad_program(j0, jend, L, nz, openb, wide=True, wave=0):
     """Wavefront `wave`'s action list in the four-stage kernel (quad_body):
     ('pub', ring, row) / ('read', ring, row) on the device rings (wide only),
     ('lpub', ring, row) / ('lread', ring, row) on the block's LDS rings,
-    ('spub', ring, row) / ('sread', ring, row) on the LDS staging buffer of
-    the partner's partials (wide only; wavefront r < 4 stages ring r),
+    ('spub', ring, row, k) / ('sread', ring, row, k) on the LDS staging
+    buffer of partner k's partials (wide only; wavefront r < 4 stages ring
+    r from partner 1),
     ('bar',)."""
     ibeg, iend = j0 - 6, j0 + L + 6
 
@@ -55,7 +62,7 @@ def quad_program(j0, jend, L, nz, openb, wide=True, wave=0):
 
     def stage(row):  # the staging wavefront: check the partner's words, write the buffer
         if wide and wave < 4 and need(wave, row):
-            return [("read", wave, row), ("spub", wave, row)]
+            return [("read", wave, row), ("spub", wave, row, 1)]
         return []
 
     prog = [("lpub", 0, ibeg), ("lpub", 0, ibeg + 1)]
@@ -72,7 +79,7 @@ def quad_program(j0, jend, L, nz, openb, wide=True, wave=0):
         # the top of the z step
         prog += [("lread", s, c[s]) for s in range(4) if need(s, c[s])]
         if wide:
-            prog += [("sread", s, c[s]) for s in range(4) if need(s, c[s])]
+            prog += [("sread", s, c[s], 1) for s in range(4) if need(s, c[s])]
         prog.append(("lpub", 0, i + 2))
         if wide:
             prog.append(("pub", 0, i + 2))
@@ -85,10 +92,17 @@ def quad_program(j0, jend, L, nz, openb, wide=True, wave=0):
     return prog
 
 
-def pair_program(j0, jend, L, nz, openb, fd, wide=True):
-    """One wavefront's action list in the two-stage kernel (pair_body)."""
+def pair_program(j0, jend, L, nz, openb, fd, wide=True, wave=0, nb=2):
+    """Wavefront `wave`'s action list in the two-stage kernel (pair_body),
+    in a team of nb blocks: wavefront w < 2 (nb - 1) stages ring
+    w // (nb - 1) from partner k = w % (nb - 1) + 1 (('read', ring, row, k)
+    on partner block role ^ k's device ring, then ('spub', ring, row, k));
+    every wavefront reads the staged rows it needs (('sread', ...))."""
     ibeg, iend = j0 - 5, j0 + L + 4
     last = ibeg + 2 * ((iend - ibeg + 1) // 2) if fd else iend  # FD: pairs of iterations
+    nx = nb - 1
+    lrg, lk = wave // nx, wave % nx + 1
+    loader = wide and wave < 2 * nx
 
     def need_a(r):  # the first stage's cells (and xneed_a)
         return j0 - 4 <= r < jend + 4 and not (openb and (r < 0 or r >= nz))
@@ -103,22 +117,28 @@ def pair_program(j0, jend, L, nz, openb, fd, wide=True):
         prog.append(("lpub", 0, i + 4))
         if wide:
             prog.append(("pub", 0, i + 4))
-        if ((i - ibeg) & 3) == 0:  # a barrier every kSync = 4 iterations
+        if wide or ((i - ibeg) & 3) == 0:  # kSync = 4 in single-block teams
             prog.append(("bar",))
         jb = i - 4
-        # both stages' partials from LDS, right after the barrier
+        # both stages' partials from LDS, right after the barrier, and the
+        # partners' staged partials of the first stage's row
         if need_a(i):
             prog.append(("lread", 0, i))
         if need_b(jb):
             prog.append(("lread", 1, jb))
         if wide and need_a(i):
-            prog.append(("read", 0, i))
+            prog += [("sread", 0, i, k) for k in range(1, nb)]
         if need_a(i):
             prog.append(("lpub", 1, i))
         if wide:
             prog.append(("pub", 1, i))
         if wide and need_b(jb):
-            prog.append(("read", 1, jb))
+            prog += [("sread", 1, jb, k) for k in range(1, nb)]
+        # the loader: check the next iteration's row, stage it
+        if loader:
+            row = i + 1 if lrg == 0 else i - 3
+            if (need_a if lrg == 0 else need_b)(row):
+                prog += [("read", lrg, row, lk), ("spub", lrg, row, lk)]
     return prog
 
 
@@ -134,13 +154,13 @@ def run_team(programs, nb, kring, rng, dev_kring=None):
     bar_count = {a: 0 for a in agents}        # barriers passed
     slot = {}                                 # (block, wave, ring, row % kring) -> row
     lds = {}                                  # (block, wave, ring, row % kring) -> (row, barriers)
-    stg = {}                                  # (block, ring, row % 2) -> (row, barriers)
+    stg = {}                                  # (block, ring, k, row % 2) -> (row, barriers)
     # LDS reads still to come, per (kind, block, ring, row): how many wavefronts will read it
     pending = {}
     for a in agents:
         for op in programs[a]:
             if op[0] in ("lread", "sread"):
-                key = (op[0], a[0], op[1], op[2])
+                key = (op[0], a[0]) + op[1:]
                 pending[key] = pending.get(key, 0) + 1
     while True:
         runnable = []
@@ -158,9 +178,10 @@ def run_team(programs, nb, kring, rng, dev_kring=None):
                         bar_count[(b, v)] == bar_count[a]) for v in waves[b]):
                     runnable.append(a)
             elif op[0] == "read":
-                _, ring, row = op
+                ring, row = op[1], op[2]
                 ok = True
-                for pb in range(nb):
+                # ('read', ring, row, k): partner block b ^ k only
+                for pb in (range(nb) if len(op) == 3 else (b ^ op[3],)):
                     if pb == b:
                         continue
                     for v in waves[pb]:
@@ -195,21 +216,21 @@ def run_team(programs, nb, kring, rng, dev_kring=None):
                         f"before every wavefront read it")
             lds[(b, w, ring, row % kring)] = (row, bar_count[a])
         elif op[0] == "spub":
-            _, ring, row = op
-            old = stg.get((b, ring, row % 2))
-            if old is not None and pending.get(("sread", b, ring, old[0]), 0) > 0:
+            _, ring, row, k = op
+            old = stg.get((b, ring, k, row % 2))
+            if old is not None and pending.get(("sread", b, ring, old[0], k), 0) > 0:
                 return (f"block {b} wave {w} rewrote staged ring {ring} row {old[0]} with row "
                         f"{row} before every wavefront read it")
-            stg[(b, ring, row % 2)] = (row, bar_count[a])
+            stg[(b, ring, k, row % 2)] = (row, bar_count[a])
         elif op[0] == "sread":
-            _, ring, row = op
-            have = stg.get((b, ring, row % 2))
+            _, ring, row, k = op
+            have = stg.get((b, ring, k, row % 2))
             if have is None or have[0] != row:
                 return f"block {b} wave {w} read staged ring {ring} row {row} holding {have}"
             if have[1] >= bar_count[a]:
                 return (f"block {b} wave {w} read staged ring {ring} row {row} with no barrier "
                         f"after it was written")
-            pending[("sread", b, ring, row)] -= 1
+            pending[("sread", b, ring, row, k)] -= 1
         elif op[0] == "lread":
             _, ring, row = op
             for v in waves[b]:
@@ -224,8 +245,14 @@ def run_team(programs, nb, kring, rng, dev_kring=None):
         pc[a] += 1
 
 
-def team_ranges(nz, nteams):
-    for t in range(nteams):
+def team_ranges(nz, nteams, sample=False):
+    """The teams' z ranges (j0, jend, L); sample: the first three, the last
+    three and two in between (the open boundary's ghost rows lie in the
+    first and last teams; the others differ only by a shift)."""
+    ts = range(nteams)
+    if sample and nteams > 8:
+        ts = sorted({0, 1, 2, nteams // 3, 2 * nteams // 3, nteams - 3, nteams - 2, nteams - 1})
+    for t in ts:
         yield t * nz // nteams, (t + 1) * nz // nteams, -(-nz // nteams)
 
 
@@ -249,9 +276,9 @@ def test_quad_exchange_never_overwrites_or_deadlocks(nz, nteams, openb):
 def test_pair_exchange_never_overwrites_or_deadlocks(nz, nteams, openb, fd):
     rng = random.Random(nz * 131 + nteams * 7 + openb * 3 + fd)
     for nb in (2, 4):
-        for j0, jend, L in team_ranges(nz, nteams):
-            prog = pair_program(j0, jend, L, nz, openb, fd)
-            programs = {(b, w): prog for b in range(nb) for w in range(W)}
+        for j0, jend, L in team_ranges(nz, nteams, sample=nb == 4):
+            programs = {(b, w): pair_program(j0, jend, L, nz, openb, fd, wave=w, nb=nb)
+                        for b in range(nb) for w in range(pair_waves(nb))}
             for _ in range(2):
                 assert run_team(programs, nb, 16, rng) is None, (nb, nz, nteams, j0, jend)
 
@@ -299,8 +326,8 @@ def test_model_detects_a_too_shallow_device_ring(kind):
                     for w in range(WQ)}
         kring = 8
     else:
-        prog = pair_program(0, 64, 64, 64, False, False)
-        programs = {(b, w): prog for b in range(2) for w in range(W)}
+        programs = {(b, w): pair_program(0, 64, 64, 64, False, False, wave=w)
+                    for b in range(2) for w in range(pair_waves(2))}
         kring = 16
     failures = [run_team(programs, 2, kring, rng, dev_kring=2) for _ in range(20)]
     assert any(f is not None and "overwrote" in f for f in failures)

@@ -726,9 +726,9 @@ int cosenu_create(const cosenu_params *p, cosenu_ctx **out)
             // the team kernels hold a team of VP/32 wavefronts in one block
             // of at most 8: rows of 512 bins with FD and 3rd-order
             // dissipation take four stages per launch with a team over 2
-            // blocks (nz = 16384, nvz = 512: 0.405 ms per step against 0.90
-            // ms for two-stage launches; open, nvz = 480: 0.62 against
-            // 0.98), other rows of 512 and 1024 bins two stages per launch
+            // blocks (nz = 16384, nvz = 512: 0.405 ms per step against 0.84
+            // ms for two-stage launches; open, nvz = 480: 0.62 ms), other
+            // rows of 512 and 1024 bins two stages per launch
             // with a team over 2 or 4 blocks, other wide grids the
             // barrier-free kernel
             c->variant =
